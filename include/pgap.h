@@ -1,0 +1,156 @@
+/*
+ * pgap.h -- C ABI of the MI355X PRP engine for prime-gap searches.
+ *
+ * The engine replaces the tester stage of sethtroisi/prime-gap:
+ *   - the in-process seam test_runner_t<params>(n, rounds) / run_test(tests,
+ *     results) (miller_rabin.h:317-383), called from run_gpu_thread
+ *     (gap_test_gpu.cu:232-268), by pgap_create + pgap_prp_batch;
+ *   - the per-m scan state machine of load_batch_thread (gap_test_gpu.cu:
+ *     323-655) and test_interval_cpu (gap_test_common.cpp:128-181), by the
+ *     streaming calls pgap_scan_*.
+ *
+ * Conventions:
+ *   - numbers are little-endian 32-bit words, the layout miller_rabin.h:37-52
+ *     exports from mpz_t (mpz_export(..., -1, 4, 0, 0)); row-major
+ *     [count][limbs] arrays;
+ *   - every call returns a pgap_status; the engine never calls exit() (the
+ *     reference's CUDA_CHECK/CGBN_CHECK exit(1), miller_rabin.h:55-94);
+ *     pgap_last_error() gives the message of the last failure;
+ *   - inputs are copied before a call returns; the caller keeps ownership;
+ *   - one engine serves one submitting thread plus one polling thread.
+ */
+#ifndef PGAP_H
+#define PGAP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pgap_engine pgap_engine;
+
+typedef enum {
+    PGAP_OK = 0,
+    PGAP_E_ARG = -1,        /* bad argument (null pointer, zero limbs, ...) */
+    PGAP_E_DEVICE = -2,     /* HIP runtime error, or the HIP code object is missing */
+    PGAP_E_OOM = -3,        /* device allocation failed / admission refused by the HBM budget */
+    PGAP_E_TOO_LARGE = -4,  /* number wider than the largest kernel variant */
+    PGAP_E_STATE = -5       /* call not valid in the engine's current state */
+} pgap_status;
+
+/* The test a batch runs.  FERMAT2 is the reference kernel's ROUNDS=1 use
+ * (gap_test_gpu.cu:81); SPRP runs a strong-probable-prime test for each base
+ * in order (generate_primes order 2, 3, 5, ..., miller_rabin.h:269-286) and a
+ * number passes when every base passes (results[i] = passed == rounds,
+ * miller_rabin.h:381). */
+typedef enum { PGAP_FERMAT2 = 0, PGAP_SPRP = 1 } pgap_test;
+
+typedef struct {
+    int device;               /* HIP device ordinal (the reference hard-codes 0, miller_rabin.h:326) */
+    size_t hbm_budget;        /* bytes the streaming scan may keep resident; 0 = 1/4 of free HBM */
+    const char *tuning_path;  /* autotuner table written by pgap_autotune(); NULL = built-in defaults */
+    int radix;                /* force 28 or 32 (0 = tuned) */
+    int lanes_per_number;     /* force lanes per number (0 = tuned) */
+    int waves_per_simd;       /* force an occupancy target (0 = tuned) */
+} pgap_options;
+
+/* Engine lifetime: owns the stream, device buffers and streaming state.
+ * Mirrors test_runner_t's constructor/destructor (miller_rabin.h:317-339). */
+pgap_status pgap_create(const pgap_options *opt, pgap_engine **out);
+void pgap_destroy(pgap_engine *e);
+const char *pgap_last_error(const pgap_engine *e);
+/* Library version string, e.g. "pgap 0.1 gfx950". */
+const char *pgap_version(void);
+
+/* ---- batch mode ------------------------------------------------------- */
+
+/* Test count numbers of `limbs` words each (host memory, row-major) and
+ * write verdict[i] in {0,1}.  Replaces test_runner_t::run_test
+ * (miller_rabin.h:341-383), without its n-per-batch ceiling (:345-348). */
+pgap_status pgap_prp_batch(pgap_engine *e, uint32_t limbs, const uint32_t *n, size_t count,
+                           pgap_test kind, const uint32_t *bases, uint32_t nbases, uint8_t *verdict);
+
+/* Same on device-resident buffers (d_n, d_verdict are device pointers),
+ * asynchronous on the engine's stream; `max_bits` is an upper bound of the
+ * numbers' bit length (0 = let the engine measure it).  pgap_sync() waits. */
+pgap_status pgap_prp_batch_device(pgap_engine *e, uint32_t limbs, const uint32_t *d_n, size_t count,
+                                  pgap_test kind, const uint32_t *bases, uint32_t nbases,
+                                  uint8_t *d_verdict, uint32_t max_bits);
+
+/* Mixed widths in one persistent launch.  Number i has limbs[i] words
+ * starting at words + offset[i].  Numbers are grouped into size buckets on
+ * the device and one work-queue kernel serves every bucket. */
+pgap_status pgap_prp_batch_mixed(pgap_engine *e, const uint32_t *words, const uint64_t *offset,
+                                 const uint32_t *limbs, size_t count, pgap_test kind,
+                                 const uint32_t *bases, uint32_t nbases, uint8_t *verdict);
+
+pgap_status pgap_sync(pgap_engine *e);
+
+/* Timing of the last PRP launch: device time measured with HIP events on
+ * the engine's stream, and the kernel variant used. */
+typedef struct {
+    double kernel_ms;
+    uint32_t digits;            /* digits per number of the variant */
+    uint32_t radix_bits;        /* 28 or 32 */
+    uint32_t lanes_per_number;
+    uint32_t launches;
+    uint64_t tests;
+} pgap_launch_info;
+pgap_status pgap_last_launch(const pgap_engine *e, pgap_launch_info *info);
+
+/* ---- streaming scan (serving) ----------------------------------------- */
+
+/* A family is K = P#/d (gap_common.cpp:116-121); registering it once lets
+ * every request name only m. */
+pgap_status pgap_family(pgap_engine *e, uint32_t P, uint32_t D, uint32_t sieve_length, int *family_id);
+
+typedef struct {
+    int family;
+    uint64_t m;
+    const int32_t *prev;  /* positive distances below m*K, increasing (outward) */
+    uint32_t n_prev;
+    const int32_t *next;  /* positive distances above m*K, increasing (outward) */
+    uint32_t n_next;
+    uint64_t tag;         /* caller cookie, echoed in the row */
+} pgap_scan_req;
+
+enum {
+    PGAP_ROW_PREV_FALLBACK = 1,  /* no PRP among prev unknowns: searched beyond SL */
+    PGAP_ROW_NEXT_FALLBACK = 2,
+};
+
+/* One finished m: the row test_interval_cpu produces (gap_test_common.cpp:
+ * 128-181).  p_tests / n_tests count tests up to and including the first PRP
+ * on each side, as s_total_prp_tests does; speculative tests beyond it are
+ * not counted. */
+typedef struct {
+    uint64_t m, tag;
+    int32_t prev_p, next_p;
+    uint32_t p_tests, n_tests;
+    uint32_t flags;
+    uint32_t family;
+    double t_submit, t_done;  /* seconds on the engine's steady clock */
+} pgap_scan_row;
+
+/* Submit one m.  PGAP_E_OOM means the HBM budget has no room right now:
+ * poll, then retry (admission control). */
+pgap_status pgap_scan_submit(pgap_engine *e, const pgap_scan_req *req);
+/* Drive the device and collect finished rows (up to cap).  wait != 0 blocks
+ * until at least one row is ready or nothing is in flight. */
+pgap_status pgap_scan_poll(pgap_engine *e, pgap_scan_row *rows, size_t cap, size_t *n_out, int wait);
+/* Number of m's submitted and not yet returned. */
+size_t pgap_scan_in_flight(const pgap_engine *e);
+
+/* Streaming knobs: speculative depth per side (tests in flight per side),
+ * Miller-Rabin confirmation rounds on each side's first PRP (bases 3, 5, ...),
+ * and the confirmation's effect: a found PRP failing confirmation is treated
+ * as composite and the scan continues. */
+pgap_status pgap_scan_config(pgap_engine *e, uint32_t depth, uint32_t confirm_rounds);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,350 @@
+/*
+ * oracle.c -- TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * GMP 6.2.1 restatement of the prime-gap tester path.  Never linked into the
+ * product library; loaded by tests/, smoke() and bench.py's cpu_baseline.
+ */
+#include "oracle.h"
+
+#include <gmp.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+static void import_limbs(mpz_t z, const uint32_t *n, int limbs) {
+    /* 32-bit little-endian words, as miller_rabin.h:37-52 exports them. */
+    mpz_import(z, (size_t)limbs, -1, sizeof(uint32_t), 0, 0, n);
+}
+
+static int export_limbs(const mpz_t z, uint32_t *out, int cap) {
+    size_t cnt = (mpz_sizeinbase(z, 2) + 31) / 32;
+    if (mpz_sgn(z) == 0) cnt = 0;
+    if ((int)cnt > cap) return -1;
+    memset(out, 0, sizeof(uint32_t) * (size_t)cap);
+    size_t written = 0;
+    mpz_export(out, &written, -1, sizeof(uint32_t), 0, 0, z);
+    return (int)cnt;
+}
+
+/* Fermat base 2: the test the reference GPU kernel runs with ROUNDS=1
+ * (gap_test_gpu.cu:81, miller_rabin.h:157-211), stated textbook-style as
+ * mpz_powm(2, N-1, N) == 1. */
+static int fermat2_z(const mpz_t N, mpz_t e, mpz_t r) {
+    if (mpz_cmp_ui(N, 1) <= 0) return 0;
+    mpz_sub_ui(e, N, 1);
+    mpz_set_ui(r, 2);
+    mpz_powm(r, r, e, N);
+    return mpz_cmp_ui(r, 1) == 0;
+}
+
+/* Strong probable prime to `base`, textbook Miller-Rabin: N-1 = d*2^s,
+ * x = base^d; pass if x == 1 or x == N-1, or x^(2^j) == N-1 for some
+ * 0 < j < s.  miller_rabin.h:213-245 allows one squaring more than this; the
+ * engine follows the textbook form, pinned here. */
+static int sprp_z(const mpz_t N, uint32_t base, mpz_t d, mpz_t x, mpz_t nm1) {
+    if (mpz_cmp_ui(N, 3) < 0 || mpz_even_p(N)) return 0;
+    mpz_sub_ui(nm1, N, 1);
+    mp_bitcnt_t s = mpz_scan1(nm1, 0);
+    mpz_tdiv_q_2exp(d, nm1, s);
+    mpz_set_ui(x, base);
+    mpz_powm(x, x, d, N);
+    if (mpz_cmp_ui(x, 1) == 0 || mpz_cmp(x, nm1) == 0) return 1;
+    for (mp_bitcnt_t j = 1; j < s; j++) {
+        mpz_mul(x, x, x);
+        mpz_mod(x, x, N);
+        if (mpz_cmp(x, nm1) == 0) return 1;
+        if (mpz_cmp_ui(x, 1) == 0) return 0;
+    }
+    return 0;
+}
+
+int or_fermat2(const uint32_t *n, int limbs) {
+    mpz_t N, e, r;
+    mpz_inits(N, e, r, NULL);
+    import_limbs(N, n, limbs);
+    int v = fermat2_z(N, e, r);
+    mpz_clears(N, e, r, NULL);
+    return v;
+}
+
+int or_sprp(const uint32_t *n, int limbs, uint32_t base) {
+    mpz_t N, d, x, nm1;
+    mpz_inits(N, d, x, nm1, NULL);
+    import_limbs(N, n, limbs);
+    int v = sprp_z(N, base, d, x, nm1);
+    mpz_clears(N, d, x, nm1, NULL);
+    return v;
+}
+
+/* gap_test_common.cpp:144,154: mpz_probab_prime_p(N, 25). */
+int or_probab_prime(const uint32_t *n, int limbs, int reps) {
+    mpz_t N;
+    mpz_init(N);
+    import_limbs(N, n, limbs);
+    int v = mpz_probab_prime_p(N, reps);
+    mpz_clear(N);
+    return v;
+}
+
+typedef struct {
+    const uint32_t *n;
+    size_t lo, hi;
+    int limbs, kind, nbases;
+    const uint32_t *bases;
+    uint8_t *out;
+} verdict_job;
+
+static void *verdict_worker(void *arg) {
+    verdict_job *j = (verdict_job *)arg;
+    mpz_t N, a, b, c;
+    mpz_inits(N, a, b, c, NULL);
+    for (size_t i = j->lo; i < j->hi; i++) {
+        import_limbs(N, j->n + i * (size_t)j->limbs, j->limbs);
+        int v;
+        if (j->kind == 0) {
+            v = fermat2_z(N, a, b);
+        } else {
+            v = 1;
+            for (int k = 0; k < j->nbases && v; k++) v = sprp_z(N, j->bases[k], a, b, c);
+        }
+        if (j->out) j->out[i] = (uint8_t)v;
+    }
+    mpz_clears(N, a, b, c, NULL);
+    return NULL;
+}
+
+void or_verdicts(const uint32_t *n, size_t count, int limbs, int kind,
+                 const uint32_t *bases, int nbases, uint8_t *out, int threads) {
+    if (threads <= 0) threads = 1;
+    if ((size_t)threads > count) threads = count ? (int)count : 1;
+    pthread_t *tid = (pthread_t *)calloc((size_t)threads, sizeof(pthread_t));
+    verdict_job *jobs = (verdict_job *)calloc((size_t)threads, sizeof(verdict_job));
+    for (int t = 0; t < threads; t++) {
+        jobs[t].n = n; jobs[t].limbs = limbs; jobs[t].kind = kind;
+        jobs[t].bases = bases; jobs[t].nbases = nbases; jobs[t].out = out;
+        jobs[t].lo = count * (size_t)t / (size_t)threads;
+        jobs[t].hi = count * (size_t)(t + 1) / (size_t)threads;
+        pthread_create(&tid[t], NULL, verdict_worker, &jobs[t]);
+    }
+    for (int t = 0; t < threads; t++) pthread_join(tid[t], NULL);
+    free(tid);
+    free(jobs);
+}
+
+size_t or_bench_verdicts(const uint32_t *n, size_t count, int limbs, int kind, int threads) {
+    static const uint32_t mr_bases[2] = {3, 5};
+    or_verdicts(n, count, limbs, kind, mr_bases, 2, NULL, threads);
+    return count;
+}
+
+/* gap_common.cpp:116-121 */
+static int init_K(mpz_t K, uint32_t P, uint32_t D) {
+    mpz_primorial_ui(K, P);
+    if (D == 0 || !mpz_divisible_ui_p(K, D)) return -1;
+    mpz_divexact_ui(K, K, D);
+    return 0;
+}
+
+int or_K(uint32_t P, uint32_t D, uint32_t *out, int cap) {
+    mpz_t K;
+    mpz_init(K);
+    int r = init_K(K, P, D);
+    if (r == 0) r = export_limbs(K, out, cap);
+    mpz_clear(K);
+    return r;
+}
+
+/* gap_common.cpp:100-104 */
+double or_K_log(uint32_t P, uint32_t D) {
+    mpz_t K;
+    mpz_init(K);
+    if (init_K(K, P, D) != 0) { mpz_clear(K); return -1.0; }
+    long exp;
+    double mantis = mpz_get_d_2exp(&exp, K);
+    double v = log(mantis) + log(2) * exp;
+    mpz_clear(K);
+    return v;
+}
+
+static void center_of(mpz_t c, uint32_t P, uint32_t D, uint64_t m) {
+    init_K(c, P, D);
+    mpz_t mm;
+    mpz_init(mm);
+    mpz_import(mm, 1, -1, sizeof(uint64_t), 0, 0, &m);
+    mpz_mul(c, c, mm);
+    mpz_clear(mm);
+}
+
+static void add_si64(mpz_t z, int64_t x) {
+    if (x >= 0) {
+        mpz_t t; mpz_init(t);
+        uint64_t u = (uint64_t)x;
+        mpz_import(t, 1, -1, sizeof(uint64_t), 0, 0, &u);
+        mpz_add(z, z, t);
+        mpz_clear(t);
+    } else {
+        mpz_t t; mpz_init(t);
+        uint64_t u = (uint64_t)(-(x + 1)) + 1u;
+        mpz_import(t, 1, -1, sizeof(uint64_t), 0, 0, &u);
+        mpz_sub(z, z, t);
+        mpz_clear(t);
+    }
+}
+
+int or_mkx(uint32_t P, uint32_t D, uint64_t m, int64_t x, uint32_t *out, int cap) {
+    mpz_t c;
+    mpz_init(c);
+    center_of(c, P, D, m);
+    add_si64(c, x);
+    int r = (mpz_sgn(c) < 0) ? -1 : export_limbs(c, out, cap);
+    mpz_clear(c);
+    return r;
+}
+
+/* Largest prime strictly below z (z > 2); GMP 6.2.1 has no mpz_prevprime. */
+static void prevprime_z(mpz_t r, const mpz_t z) {
+    mpz_sub_ui(r, z, 1);
+    if (mpz_cmp_ui(r, 2) <= 0) return;
+    if (mpz_even_p(r)) mpz_sub_ui(r, r, 1);
+    while (!mpz_probab_prime_p(r, 25)) mpz_sub_ui(r, r, 2);
+}
+
+int64_t or_prev_prime_dist(uint32_t P, uint32_t D, uint64_t m, int64_t start) {
+    mpz_t c, t;
+    mpz_inits(c, t, NULL);
+    center_of(c, P, D, m);
+    mpz_set(t, c);
+    add_si64(t, -start);
+    prevprime_z(t, t);
+    mpz_sub(t, c, t);
+    int64_t d = (int64_t)mpz_get_ui(t);
+    mpz_clears(c, t, NULL);
+    return d;
+}
+
+int64_t or_next_prime_dist(uint32_t P, uint32_t D, uint64_t m, int64_t start) {
+    mpz_t c, t;
+    mpz_inits(c, t, NULL);
+    center_of(c, P, D, m);
+    mpz_set(t, c);
+    add_si64(t, start);
+    mpz_nextprime(t, t);
+    mpz_sub(t, t, c);
+    int64_t d = (int64_t)mpz_get_ui(t);
+    mpz_clears(c, t, NULL);
+    return d;
+}
+
+/* gap_test_common.cpp:128-181 */
+void or_test_interval(uint32_t P, uint32_t D, uint64_t m, uint32_t SL,
+                      const int32_t *prev, int np, const int32_t *next, int nn,
+                      int kind, int64_t *out) {
+    mpz_t center, t, e, r;
+    mpz_inits(center, t, e, r, NULL);
+    center_of(center, P, D, m);
+    int64_t prev_p = 0, next_p = 0, tests = 0, ptests = 0, ntests = 0;
+    for (int i = 0; i < np; i++) {
+        tests++; ptests++;
+        mpz_sub_ui(t, center, (unsigned long)prev[i]);
+        int v = kind == 0 ? mpz_probab_prime_p(t, 25) : fermat2_z(t, e, r);
+        if (v) { prev_p = prev[i]; break; }
+    }
+    for (int i = 0; i < nn; i++) {
+        tests++; ntests++;
+        mpz_add_ui(t, center, (unsigned long)next[i]);
+        int v = kind == 0 ? mpz_probab_prime_p(t, 25) : fermat2_z(t, e, r);
+        if (v) { next_p = next[i]; break; }
+    }
+    int oos_prev = 0, oos_next = 0;
+    if (prev_p == 0) {
+        oos_prev = 1;
+        mpz_sub_ui(t, center, SL);
+        prevprime_z(t, t);
+        mpz_sub(t, center, t);
+        prev_p = (int64_t)mpz_get_ui(t);
+    }
+    if (next_p == 0) {
+        oos_next = 1;
+        mpz_add_ui(t, center, SL);
+        mpz_nextprime(t, t);
+        mpz_sub(t, t, center);
+        next_p = (int64_t)mpz_get_ui(t);
+    }
+    out[0] = prev_p; out[1] = next_p; out[2] = tests;
+    out[3] = oos_prev; out[4] = oos_next; out[5] = ptests; out[6] = ntests;
+    mpz_clears(center, t, e, r, NULL);
+}
+
+/* ---- sieve: the unknown set combined_sieve writes ---- */
+struct or_sieve {
+    uint32_t P, D, SL;
+    uint64_t max_prime;
+    size_t nprimes;
+    uint32_t *primes;
+    uint32_t *kmod;   /* K mod p */
+};
+
+or_sieve *or_sieve_new(uint32_t P, uint32_t D, uint32_t SL, uint64_t max_prime) {
+    if (max_prime > 4000000000ull || max_prime < 2) return NULL;
+    mpz_t K;
+    mpz_init(K);
+    if (init_K(K, P, D) != 0) { mpz_clear(K); return NULL; }
+    uint64_t half = max_prime / 2 + 1;
+    uint8_t *comp = (uint8_t *)calloc(half / 8 + 1, 1);  /* bit i <-> odd 2i+1 */
+    size_t cnt = 1;
+    for (uint64_t i = 1; i < half; i++) {
+        if (comp[i >> 3] & (1u << (i & 7))) continue;
+        uint64_t p = 2 * i + 1;
+        if (p > max_prime) break;
+        cnt++;
+        for (uint64_t j = (p * p) / 2; j < half; j += p) comp[j >> 3] |= (uint8_t)(1u << (j & 7));
+    }
+    or_sieve *s = (or_sieve *)calloc(1, sizeof(or_sieve));
+    s->P = P; s->D = D; s->SL = SL; s->max_prime = max_prime;
+    s->primes = (uint32_t *)malloc(sizeof(uint32_t) * cnt);
+    s->kmod = (uint32_t *)malloc(sizeof(uint32_t) * cnt);
+    size_t k = 0;
+    s->primes[k++] = 2;
+    for (uint64_t i = 1; i < half; i++) {
+        uint64_t p = 2 * i + 1;
+        if (p > max_prime) break;
+        if (!(comp[i >> 3] & (1u << (i & 7)))) s->primes[k++] = (uint32_t)p;
+    }
+    s->nprimes = k;
+    for (size_t i = 0; i < k; i++) s->kmod[i] = (uint32_t)mpz_fdiv_ui(K, s->primes[i]);
+    free(comp);
+    mpz_clear(K);
+    return s;
+}
+
+void or_sieve_free(or_sieve *s) {
+    if (!s) return;
+    free(s->primes);
+    free(s->kmod);
+    free(s);
+}
+
+size_t or_sieve_nprimes(const or_sieve *s) { return s ? s->nprimes : 0; }
+
+int or_sieve_m(const or_sieve *s, uint64_t m, int32_t *prev, int *np, int32_t *next, int *nn, int cap) {
+    const uint32_t SL = s->SL;
+    const uint32_t W = 2 * SL + 1;
+    uint8_t *comp = (uint8_t *)calloc(W, 1);
+    for (size_t i = 0; i < s->nprimes; i++) {
+        uint64_t p = s->primes[i];
+        uint64_t r = (uint64_t)((m % p) * s->kmod[i] % p);       /* m*K mod p */
+        uint64_t first = ((SL % p) + p - r) % p;                   /* index of x = -r */
+        for (uint64_t j = first; j < W; j += p) comp[j] = 1;
+    }
+    comp[SL] = 1; /* x = 0 is never an unknown */
+    int a = 0, b = 0;
+    for (uint32_t d = 1; d <= SL; d++) {
+        if (!comp[SL - d]) { if (a >= cap) { free(comp); return -1; } prev[a++] = (int32_t)d; }
+        if (!comp[SL + d]) { if (b >= cap) { free(comp); return -1; } next[b++] = (int32_t)d; }
+    }
+    *np = a;
+    *nn = b;
+    free(comp);
+    return 0;
+}

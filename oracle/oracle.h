@@ -1,0 +1,93 @@
+/*
+ * oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A CPU restatement, on GMP 6.2.1, of the prime-gap tester path that the HIP
+ * engine replaces.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load liboracle.so, and only as the checker or as the
+ * timed CPU baseline; the product library (libpgap.so) never links it.
+ *
+ * Each function cites the reference file:line whose behaviour it restates.
+ * How each function is pinned independently of itself:
+ *   or_fermat2 / or_sprp   : Python's own big-integer pow() (an independent
+ *                            bignum implementation) at every size, brute force
+ *                            factoring for N < 2^32, the published base-2
+ *                            Fermat / strong pseudoprime lists, Mersenne and
+ *                            Fermat numbers (tests/test_oracle.py).
+ *   or_probab_prime        : brute force below 2^32, known primes/composites.
+ *   or_prevprime/nextprime : brute force on small ranges; the record gap
+ *                            11244911*809#/1841070 (-5788, +17354) from the
+ *                            reference README.md:40-46.
+ *   or_test_interval       : README.md:265-288 rows for 907#/2190 (prev/next
+ *                            per m) and the prp-test totals printed in
+ *                            README.md:241-262 (144, 643, 1550, 2590).
+ *   or_sieve_*             : md5 15a5cbff... of the reference's own unknown
+ *                            file (misc/tests.sh:62-67) and the per-m unknown
+ *                            counts in README.md:245-262.
+ *   or_K / or_K_log        : mpz_primorial_ui against a Python product of
+ *                            primes; K_log against math.log of the integer.
+ */
+#ifndef PGAP_ORACLE_H
+#define PGAP_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Verdicts on one number given as `limbs` little-endian 32-bit words. */
+int or_fermat2(const uint32_t *n, int limbs);                 /* 2^(N-1) mod N == 1 */
+int or_sprp(const uint32_t *n, int limbs, uint32_t base);     /* strong PRP, textbook */
+int or_probab_prime(const uint32_t *n, int limbs, int reps);  /* mpz_probab_prime_p */
+
+/* Batched verdicts: n is row-major [count][limbs].  kind 0 = Fermat base 2,
+ * kind 1 = strong PRP to each base in bases[0..nbases). threads <= 0 means 1. */
+void or_verdicts(const uint32_t *n, size_t count, int limbs, int kind,
+                 const uint32_t *bases, int nbases, uint8_t *out, int threads);
+
+/* K = P#/d (gap_common.cpp:116-121).  Writes up to cap limbs, returns the
+ * number of limbs of K, or -1 if d does not divide P#. */
+int or_K(uint32_t P, uint32_t D, uint32_t *out, int cap);
+/* ln(K) computed exactly as gap_common.cpp:100-104 (mpz_get_d_2exp). */
+double or_K_log(uint32_t P, uint32_t D);
+
+/* N = m*K + x as limbs (x may be negative); returns limb count or -1. */
+int or_mkx(uint32_t P, uint32_t D, uint64_t m, int64_t x, uint32_t *out, int cap);
+
+/* Distances from centre = m*K to the previous prime below centre - start and
+ * the next prime above centre + start, restating the fallback of
+ * gap_test_common.cpp:160-178 (mpz_prevprime / mpz_nextprime).  The GMP here
+ * (6.2.1) lacks mpz_prevprime, so the previous prime is a downward scan over
+ * mpz_probab_prime_p(., 25). */
+int64_t or_prev_prime_dist(uint32_t P, uint32_t D, uint64_t m, int64_t start);
+int64_t or_next_prime_dist(uint32_t P, uint32_t D, uint64_t m, int64_t start);
+
+/* One m of gap_test_simple (gap_test_common.cpp:128-181): scan prev then next
+ * unknown distances with mpz_probab_prime_p(N, 25), first PRP wins, fall back
+ * beyond SL.  out[0]=prev_p out[1]=next_p out[2]=prp tests (including hits)
+ * out[3]=prev fell back out[4]=next fell back out[5]=prev tests out[6]=next
+ * tests.  kind 0 uses mpz_probab_prime_p(N,25) as the reference does; kind 1
+ * uses Fermat base 2 (the engine's test) for the in-sieve scan. */
+void or_test_interval(uint32_t P, uint32_t D, uint64_t m, uint32_t SL,
+                      const int32_t *prev, int np, const int32_t *next, int nn,
+                      int kind, int64_t *out);
+
+/* Sieve: unknowns of m*K + x for x in [-SL, SL] \ {0} free of every prime
+ * <= max_prime, the set combined_sieve writes (combined_sieve.cpp:1261-1445).
+ * prev gets positive distances in increasing order, next likewise. */
+typedef struct or_sieve or_sieve;
+or_sieve *or_sieve_new(uint32_t P, uint32_t D, uint32_t SL, uint64_t max_prime);
+void or_sieve_free(or_sieve *s);
+int or_sieve_m(const or_sieve *s, uint64_t m, int32_t *prev, int *np, int32_t *next, int *nn, int cap);
+size_t or_sieve_nprimes(const or_sieve *s);
+
+/* CPU baselines, timed by the caller: a static partition over `threads`
+ * pthreads.  Return the number of items done.  kind as in or_verdicts. */
+size_t or_bench_verdicts(const uint32_t *n, size_t count, int limbs, int kind, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,328 @@
+// engine.hip -- the C ABI of include/pgap.h: engine lifetime, batch PRP
+// tests and launch bookkeeping.  The streaming scan lives in scan.hip.
+//
+// Reference seam replaced: test_runner_t (miller_rabin.h:293-383).  Unlike
+// the reference, errors are returned as pgap_status (never exit()), device
+// buffers are reused across calls, and timing comes from HIP events on the
+// engine's own stream.
+#include "engine_impl.h"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+namespace pgap {
+
+// ---- variant registry ---------------------------------------------------
+
+static std::vector<Variant> &registry() {
+    static std::vector<Variant> v;
+    return v;
+}
+
+void register_variant(const Variant &v) { registry().push_back(v); }
+size_t variant_count() { return registry().size(); }
+const Variant *variant_at(size_t i) { return i < registry().size() ? &registry()[i] : nullptr; }
+
+const Variant *find_variant(int kind, int bits, int radix, int lanes) {
+    const Variant *best = nullptr;
+    for (const Variant &v : registry()) {
+        if (v.kind != kind || v.max_bits < bits) continue;
+        if (radix && v.radix != radix) continue;
+        if (lanes && v.lanes != lanes) continue;
+        if (!best || v.max_bits < best->max_bits ||
+            (v.max_bits == best->max_bits && v.lanes < best->lanes))
+            best = &v;
+    }
+    return best;
+}
+
+// ---- small device helpers ----------------------------------------------
+
+// Max bit length over a row-major batch (atomicMax into *out).
+__global__ void max_bits_kernel(const uint32_t *__restrict__ n, uint32_t limbs, uint32_t count,
+                                uint32_t *__restrict__ out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    uint32_t b = 0;
+    if (i < count) {
+        const uint32_t *w = n + (size_t)i * limbs;
+        for (int j = (int)limbs - 1; j >= 0; --j) {
+            if (w[j]) {
+                b = 32u * (uint32_t)j + (32u - (uint32_t)__builtin_clz(w[j]));
+                break;
+            }
+        }
+    }
+    // wave-level max first, then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        uint32_t o = (uint32_t)__shfl_down((int)b, off, 64);
+        b = b > o ? b : o;
+    }
+    if ((threadIdx.x & 63) == 0 && b) atomicMax(out, b);
+}
+
+}  // namespace pgap
+
+using namespace pgap;
+
+// ---- engine -------------------------------------------------------------
+
+void pgap_engine::set_error(const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    std::lock_guard<std::mutex> g(err_mu);
+    err = buf;
+}
+
+pgap_status pgap_engine::hip_fail(hipError_t e, const char *what) {
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return PGAP_E_DEVICE;
+}
+
+pgap_status pgap_engine::ensure(DevBuf &b, size_t bytes, const char *what) {
+    if (b.cap >= bytes) return PGAP_OK;
+    if (b.p) {
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+        hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    size_t want = std::max(bytes, (size_t)4096);
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("hipMalloc(%zu) for %s: %s", want, what, hipGetErrorString(e));
+        return PGAP_E_OOM;
+    }
+    b.cap = want;
+    return PGAP_OK;
+}
+
+pgap_status pgap_engine::pick(int kind, uint32_t bits, const Variant **out) {
+    const Variant *v = find_variant(kind, (int)bits, opt.radix, opt.lanes_per_number);
+    if (!v && (opt.radix || opt.lanes_per_number)) v = find_variant(kind, (int)bits, 0, 0);
+    if (!v) {
+        set_error("no kernel variant for %u-bit numbers (kind %d)", bits, kind);
+        return PGAP_E_TOO_LARGE;
+    }
+    *out = v;
+    return PGAP_OK;
+}
+
+pgap_status pgap_engine::launch_batch(const Variant *v, const uint32_t *d_n, uint32_t limbs, size_t count,
+                                      const Bases &bases, uint8_t *d_out) {
+    hipError_t e = hipEventRecord(ev0, stream);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    // Grid dimensions are 32-bit: split very large batches.
+    const size_t chunk = (size_t)1 << 30;
+    uint32_t launches = 0;
+    for (size_t off = 0; off < count; off += chunk) {
+        const uint32_t c = (uint32_t)std::min(chunk, count - off);
+        e = v->launch(d_n + off * limbs, limbs, c, bases, d_out + off, opt.waves_per_simd, stream);
+        if (e != hipSuccess) return hip_fail(e, "kernel launch");
+        ++launches;
+    }
+    e = hipEventRecord(ev1, stream);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    last.digits = (uint32_t)v->digits;
+    last.radix_bits = (uint32_t)v->radix;
+    last.lanes_per_number = (uint32_t)v->lanes;
+    last.launches = launches;
+    last.tests = count;
+    last.kernel_ms = -1.0;
+    timing_pending = true;
+    return PGAP_OK;
+}
+
+void pgap_engine::collect_timing() {
+    if (!timing_pending) return;
+    float ms = 0.f;
+    if (hipEventSynchronize(ev1) == hipSuccess && hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess)
+        last.kernel_ms = ms;
+    timing_pending = false;
+}
+
+static pgap_status check_bases(pgap_engine *e, pgap_test kind, const uint32_t *bases, uint32_t nbases,
+                               Bases *out) {
+    memset(out, 0, sizeof *out);
+    if (kind == PGAP_FERMAT2) return PGAP_OK;
+    if (kind != PGAP_SPRP) {
+        e->set_error("unknown test kind %d", (int)kind);
+        return PGAP_E_ARG;
+    }
+    if (nbases == 0 || nbases > (uint32_t)kMaxBases || !bases) {
+        e->set_error("SPRP needs 1..%d bases (got %u)", kMaxBases, nbases);
+        return PGAP_E_ARG;
+    }
+    out->n = nbases;
+    for (uint32_t i = 0; i < nbases; ++i) {
+        if (bases[i] < 2) {
+            e->set_error("SPRP base %u < 2", bases[i]);
+            return PGAP_E_ARG;
+        }
+        out->b[i] = bases[i];
+    }
+    return PGAP_OK;
+}
+
+static uint32_t host_max_bits(const uint32_t *n, uint32_t limbs, size_t count) {
+    uint32_t best = 0;
+    for (size_t i = 0; i < count; ++i) {
+        const uint32_t *w = n + i * limbs;
+        for (int j = (int)limbs - 1; j >= 0; --j) {
+            if (w[j]) {
+                uint32_t b = 32u * (uint32_t)j + (32u - (uint32_t)__builtin_clz(w[j]));
+                best = std::max(best, b);
+                break;
+            }
+        }
+    }
+    return best;
+}
+
+extern "C" {
+
+const char *pgap_version(void) { return "pgap 0.1 gfx950"; }
+
+pgap_status pgap_create(const pgap_options *opt, pgap_engine **out) {
+    if (!out) return PGAP_E_ARG;
+    *out = nullptr;
+    pgap_engine *e = new pgap_engine();
+    if (opt) e->opt = *opt;
+    if (variant_count() == 0) {
+        e->set_error("no HIP kernel variants registered (code objects missing)");
+        *out = e;
+        return PGAP_E_DEVICE;
+    }
+    int ndev = 0;
+    hipError_t he = hipGetDeviceCount(&ndev);
+    if (he != hipSuccess || ndev == 0) {
+        e->set_error("no HIP device: %s", hipGetErrorString(he));
+        *out = e;
+        return PGAP_E_DEVICE;
+    }
+    if (e->opt.device < 0 || e->opt.device >= ndev) {
+        e->set_error("device %d out of range (have %d)", e->opt.device, ndev);
+        *out = e;
+        return PGAP_E_ARG;
+    }
+    if ((he = hipSetDevice(e->opt.device)) != hipSuccess ||
+        (he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (he = hipEventCreate(&e->ev0)) != hipSuccess || (he = hipEventCreate(&e->ev1)) != hipSuccess) {
+        e->hip_fail(he, "engine set-up");
+        *out = e;
+        return PGAP_E_DEVICE;
+    }
+    e->ready = true;
+    *out = e;
+    return PGAP_OK;
+}
+
+void pgap_destroy(pgap_engine *e) {
+    if (!e) return;
+    if (e->ready) {
+        hipSetDevice(e->opt.device);
+        hipStreamSynchronize(e->stream);
+        e->scan_shutdown();
+        for (DevBuf *b : {&e->d_n, &e->d_out, &e->d_small}) {
+            if (b->p) hipFree(b->p);
+        }
+        hipEventDestroy(e->ev0);
+        hipEventDestroy(e->ev1);
+        hipStreamDestroy(e->stream);
+    }
+    delete e;
+}
+
+const char *pgap_last_error(const pgap_engine *e) {
+    if (!e) return "null engine";
+    std::lock_guard<std::mutex> g(e->err_mu);
+    return e->err.c_str();
+}
+
+pgap_status pgap_prp_batch(pgap_engine *e, uint32_t limbs, const uint32_t *n, size_t count, pgap_test kind,
+                           const uint32_t *bases, uint32_t nbases, uint8_t *verdict) {
+    if (!e) return PGAP_E_ARG;
+    if (!e->ready) return PGAP_E_STATE;
+    if (count == 0) return PGAP_OK;
+    if (!n || !verdict || limbs == 0) {
+        e->set_error("pgap_prp_batch: null buffer or zero limbs");
+        return PGAP_E_ARG;
+    }
+    Bases bs;
+    pgap_status st = check_bases(e, kind, bases, nbases, &bs);
+    if (st) return st;
+    hipSetDevice(e->opt.device);
+    const uint32_t bits = host_max_bits(n, limbs, count);
+    const Variant *v;
+    if ((st = e->pick((int)kind, std::max(bits, 2u), &v))) return st;
+    const size_t in_bytes = count * limbs * sizeof(uint32_t);
+    if ((st = e->ensure(e->d_n, in_bytes, "numbers")) || (st = e->ensure(e->d_out, count, "verdicts")))
+        return st;
+    hipError_t he = hipMemcpyAsync(e->d_n.p, n, in_bytes, hipMemcpyHostToDevice, e->stream);
+    if (he != hipSuccess) return e->hip_fail(he, "hipMemcpyAsync H2D");
+    if ((st = e->launch_batch(v, (const uint32_t *)e->d_n.p, limbs, count, bs, (uint8_t *)e->d_out.p)))
+        return st;
+    he = hipMemcpyAsync(verdict, e->d_out.p, count, hipMemcpyDeviceToHost, e->stream);
+    if (he != hipSuccess) return e->hip_fail(he, "hipMemcpyAsync D2H");
+    he = hipStreamSynchronize(e->stream);
+    if (he != hipSuccess) return e->hip_fail(he, "hipStreamSynchronize");
+    e->collect_timing();
+    return PGAP_OK;
+}
+
+pgap_status pgap_prp_batch_device(pgap_engine *e, uint32_t limbs, const uint32_t *d_n, size_t count,
+                                  pgap_test kind, const uint32_t *bases, uint32_t nbases, uint8_t *d_verdict,
+                                  uint32_t max_bits) {
+    if (!e) return PGAP_E_ARG;
+    if (!e->ready) return PGAP_E_STATE;
+    if (count == 0) return PGAP_OK;
+    if (!d_n || !d_verdict || limbs == 0) {
+        e->set_error("pgap_prp_batch_device: null buffer or zero limbs");
+        return PGAP_E_ARG;
+    }
+    Bases bs;
+    pgap_status st = check_bases(e, kind, bases, nbases, &bs);
+    if (st) return st;
+    hipSetDevice(e->opt.device);
+    if (max_bits == 0) {
+        if ((st = e->ensure(e->d_small, 64, "scratch"))) return st;
+        hipError_t he = hipMemsetAsync(e->d_small.p, 0, sizeof(uint32_t), e->stream);
+        if (he != hipSuccess) return e->hip_fail(he, "hipMemsetAsync");
+        for (size_t off = 0; off < count; off += ((size_t)1 << 30)) {
+            const uint32_t c = (uint32_t)std::min((size_t)1 << 30, count - off);
+            hipLaunchKernelGGL(max_bits_kernel, dim3((c + 255) / 256), dim3(256), 0, e->stream, d_n + off * limbs,
+                               limbs, c, (uint32_t *)e->d_small.p);
+        }
+        he = hipMemcpyAsync(&max_bits, e->d_small.p, sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream);
+        if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+        if (he != hipSuccess) return e->hip_fail(he, "max bits");
+    }
+    const Variant *v;
+    if ((st = e->pick((int)kind, std::max(max_bits, 2u), &v))) return st;
+    return e->launch_batch(v, d_n, limbs, count, bs, d_verdict);
+}
+
+pgap_status pgap_sync(pgap_engine *e) {
+    if (!e) return PGAP_E_ARG;
+    if (!e->ready) return PGAP_E_STATE;
+    hipError_t he = hipStreamSynchronize(e->stream);
+    if (he != hipSuccess) return e->hip_fail(he, "hipStreamSynchronize");
+    e->collect_timing();
+    return PGAP_OK;
+}
+
+pgap_status pgap_last_launch(const pgap_engine *e, pgap_launch_info *info) {
+    if (!e || !info) return PGAP_E_ARG;
+    *info = e->last;
+    return PGAP_OK;
+}
+
+}  // extern "C"

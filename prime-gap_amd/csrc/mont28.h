@@ -1,0 +1,190 @@
+// mont28.h -- one-number-per-lane Montgomery arithmetic in radix 2^28.
+//
+// A number of up to 28*L bits is held as L digits of 28 bits, each in a
+// 32-bit VGPR.  A 28x28-bit product is < 2^56, so a column of up to 255
+// products accumulates in a 64-bit register with plain v_mad_u64_u32 (the
+// 64-bit addend folded in) and never needs a carry-out: one VALU slot per
+// product.  With 32-bit digits the same column needs v_mad_u64_u32's carry-out
+// plus a v_addc per product (two slots), and on gfx950 v_mad_u64_u32 issues at
+// full rate, so the carry add costs as much as the multiply.
+//
+// Squaring is finely integrated product scanning (FIPS): column k of a^2 (the
+// cross products once, doubled, plus the diagonal) and of m*N are summed in the
+// same accumulator, m_k = (col mod 2^28) * n' is formed while the low columns
+// run, and the high columns are the result.  Lazy reduction: with N < R/16,
+// R = 2^(28L), an input < 4N gives an output < 2N, so no conditional
+// subtraction runs inside the exponentiation.
+//
+// Reference counterpart: CGBN's cgbn_mont_sqr / cgbn_mont_mul as called by
+// miller_rabin_t::powm (miller_rabin.h:157-211); the arithmetic here is
+// written from the Montgomery/FIPS method, not from CGBN.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pgap {
+namespace r28 {
+
+constexpr uint32_t MASK = (1u << 28) - 1;
+
+__device__ __forceinline__ uint64_t mad(uint32_t a, uint32_t b, uint64_t c) {
+    return (uint64_t)a * b + c;
+}
+
+// -N^{-1} mod 2^28 for odd n0 (Newton iteration, each step doubles the
+// number of correct low bits: 3 -> 6 -> 12 -> 24 -> 48).
+__device__ __forceinline__ uint32_t neg_inv28(uint32_t n0) {
+    uint32_t x = n0;                 // n0 * n0 == 1 mod 8
+    x *= 2u - n0 * x;
+    x *= 2u - n0 * x;
+    x *= 2u - n0 * x;
+    x *= 2u - n0 * x;
+    return (0u - x) & MASK;
+}
+
+// Convert W little-endian 32-bit words to L radix-2^28 digits (bits above
+// 28*L must be zero; the caller guarantees it by bucket choice).
+template <int L, int W>
+__device__ __forceinline__ void from_words(const uint32_t (&w)[W], uint32_t (&d)[L]) {
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        const int bit = 28 * j;
+        const int wi = bit >> 5, sh = bit & 31;
+        uint32_t lo = wi < W ? w[wi] : 0u;
+        uint32_t hi = (wi + 1) < W ? w[wi + 1] : 0u;
+        uint32_t v = sh ? ((lo >> sh) | (hi << (32 - sh))) : lo;
+        d[j] = v & MASK;
+    }
+}
+
+// Bit length of the value held in digits (0 for zero).
+template <int L>
+__device__ __forceinline__ int bitlen(const uint32_t (&d)[L]) {
+    int b = 0;
+#pragma unroll
+    for (int j = 0; j < L; ++j)
+        if (d[j]) b = 28 * j + (32 - __builtin_clz(d[j]));
+    return b;
+}
+
+// y = 2y mod N for y < N (digits normalised).
+template <int L>
+__device__ __forceinline__ void mod_double(uint32_t (&y)[L], const uint32_t (&n)[L]) {
+    uint32_t t[L];
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        uint32_t v = (y[j] << 1) | c;
+        c = v >> 28;
+        t[j] = v & MASK;
+    }
+    // t - N; keep it when no borrow (or when 2y overflowed the top digit).
+    uint32_t br = 0;
+    uint32_t u[L];
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        uint32_t v = t[j] - n[j] - br;
+        br = v >> 31;
+        u[j] = v & MASK;
+    }
+    const bool take = (br == 0) || (c != 0);
+#pragma unroll
+    for (int j = 0; j < L; ++j) y[j] = take ? u[j] : t[j];
+}
+
+// y = y mod N for y < 2N.
+template <int L>
+__device__ __forceinline__ void cond_sub(uint32_t (&y)[L], const uint32_t (&n)[L]) {
+    uint32_t br = 0;
+    uint32_t u[L];
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        uint32_t v = y[j] - n[j] - br;
+        br = v >> 31;
+        u[j] = v & MASK;
+    }
+#pragma unroll
+    for (int j = 0; j < L; ++j) y[j] = br ? y[j] : u[j];
+}
+
+template <int L>
+__device__ __forceinline__ bool equal(const uint32_t (&a)[L], const uint32_t (&b)[L]) {
+    uint32_t x = 0;
+#pragma unroll
+    for (int j = 0; j < L; ++j) x |= a[j] ^ b[j];
+    return x == 0;
+}
+
+// a <- a^2 * R^{-1} * 2^s mod N (s in {0,1}), lazy: a < 4N in, a < 4N out.
+// The 2^s factor is folded into the high (result) columns, so the doubling
+// step of a base-2 exponentiation costs two VALU slots per digit of output.
+template <int L>
+__device__ __forceinline__ void sqr_redc(uint32_t (&a)[L], const uint32_t (&n)[L], uint32_t np,
+                                         uint32_t s) {
+    static_assert(L >= 2 && L <= 63, "radix-2^28 column bound");
+    uint32_t m[L];
+    uint64_t T = 0;
+#pragma unroll
+    for (int k = 0; k < 2 * L - 1; ++k) {
+        const int ilo = k - (L - 1) > 0 ? k - (L - 1) : 0;
+        // cross products a_i * a_{k-i}, i < k-i
+        uint64_t X = 0;
+#pragma unroll
+        for (int i = ilo; 2 * i < k; ++i) X = mad(a[i], a[k - i], X);
+        if (k < L) {
+            T = (X << 1) + T;
+            if ((k & 1) == 0) T = mad(a[k >> 1], a[k >> 1], T);
+#pragma unroll
+            for (int i = ilo; i < k; ++i) T = mad(m[i], n[k - i], T);
+            m[k] = ((uint32_t)T * np) & MASK;
+            T = mad(m[k], n[0], T);
+            T >>= 28;
+        } else {
+            uint64_t Z = 0;
+            if ((k & 1) == 0) Z = mad(a[k >> 1], a[k >> 1], Z);
+#pragma unroll
+            for (int i = ilo; i < L; ++i) Z = mad(m[i], n[k - i], Z);
+            Z = (X << 1) + Z;
+            if (k == L) T <<= s;
+            T = (Z << s) + T;
+            // digit k-L of the result; a[k-L] and m[k-L] are dead here
+            a[k - L] = (uint32_t)T & MASK;
+            T >>= 28;
+        }
+    }
+    a[L - 1] = (uint32_t)T & MASK;
+}
+
+// c <- a * b * R^{-1} mod N, lazy (a, b < 4N, N < R/16 => c < 2N).
+template <int L>
+__device__ __forceinline__ void mul_redc(uint32_t (&c)[L], const uint32_t (&a)[L], const uint32_t (&b)[L],
+                                         const uint32_t (&n)[L], uint32_t np) {
+    uint32_t m[L];
+    uint32_t r[L];
+    uint64_t T = 0;
+#pragma unroll
+    for (int k = 0; k < 2 * L - 1; ++k) {
+        const int ilo = k - (L - 1) > 0 ? k - (L - 1) : 0;
+        const int ihi = k < L - 1 ? k : L - 1;
+#pragma unroll
+        for (int i = ilo; i <= ihi; ++i) T = mad(a[i], b[k - i], T);
+        if (k < L) {
+#pragma unroll
+            for (int i = ilo; i < k; ++i) T = mad(m[i], n[k - i], T);
+            m[k] = ((uint32_t)T * np) & MASK;
+            T = mad(m[k], n[0], T);
+            T >>= 28;
+        } else {
+#pragma unroll
+            for (int i = ilo; i < L; ++i) T = mad(m[i], n[k - i], T);
+            r[k - L] = (uint32_t)T & MASK;
+            T >>= 28;
+        }
+    }
+    r[L - 1] = (uint32_t)T & MASK;
+#pragma unroll
+    for (int j = 0; j < L; ++j) c[j] = r[j];
+}
+
+}  // namespace r28
+}  // namespace pgap

@@ -1,0 +1,46 @@
+// prp_r28_impl.h -- batch kernels for the one-number-per-lane radix-2^28
+// variants; included by the prp_r28_*.hip instantiation units.
+#pragma once
+#include "kernels.h"
+#include "prp28.h"
+
+namespace pgap {
+
+// One lane per number; 256-thread blocks (4 waves).  Input words are read
+// once per number plus one exponent word per 32 squarings, so the layout
+// (row-major per number) does not bound the kernel: it is VALU-bound.
+template <int L, int KIND>
+__global__ void __launch_bounds__(256) prp_r28_kernel(const uint32_t *__restrict__ n, uint32_t limbs,
+                                                      uint32_t count, Bases bases,
+                                                      uint8_t *__restrict__ out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t *nw = n + (size_t)i * limbs;
+    uint32_t v;
+    if (KIND == KIND_FERMAT2) {
+        v = r28::fermat2<L>(nw, limbs);
+    } else {
+        v = 1;
+        for (uint32_t k = 0; k < bases.n && v; ++k) v = r28::sprp<L>(nw, limbs, bases.b[k]);
+    }
+    out[i] = (uint8_t)v;
+}
+
+template <int L, int KIND>
+hipError_t launch_r28(const uint32_t *d_n, uint32_t limbs, uint32_t count, Bases bases, uint8_t *d_out,
+                      int /*waves_per_simd*/, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    const dim3 grid((count + 255u) / 256u);
+    hipLaunchKernelGGL((prp_r28_kernel<L, KIND>), grid, dim3(256), 0, stream, d_n, limbs, count, bases, d_out);
+    return hipGetLastError();
+}
+
+template <int... Ls>
+struct RegisterR28 {
+    RegisterR28() {
+        (register_variant(Variant{28, Ls, 1, KIND_FERMAT2, 28 * Ls - 4, &launch_r28<Ls, KIND_FERMAT2>}), ...);
+        (register_variant(Variant{28, Ls, 1, KIND_SPRP, 28 * Ls - 4, &launch_r28<Ls, KIND_SPRP>}), ...);
+    }
+};
+
+}  // namespace pgap

@@ -1,0 +1,96 @@
+"""Batch PRP parity: the HIP kernels against the GMP oracle, bit-exact."""
+import math
+import random
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import pgap
+
+pytestmark = pytest.mark.gpu
+
+
+def special_values():
+    v = [0, 1, 2, 3, 4, 5, 7, 9, 15, 341, 561, 645, 1105, 1387, 1729, 1905, 2047, 2465, 2701, 2821, 3277,
+         4033, 4369, 4371, 4681, 5461, 6601, 7957, 8321, 25326001, 3215031751, (1 << 31) - 1, (1 << 32) - 5,
+         (1 << 61) - 1, (1 << 64) - 59, (1 << 89) - 1, (1 << 127) - 1]
+    v += [(1 << p) - 1 for p in (11, 23, 29, 37, 41, 67, 521, 607, 1109, 1279)]
+    v += [(1 << (1 << n)) + 1 for n in (5, 6, 7, 8, 9, 10)]
+    return v
+
+
+def candidates(P, D, mlo, mhi, xr, count, seed):
+    rng = random.Random(seed)
+    K = ol.K(P, D)
+    out = []
+    while len(out) < count:
+        m = rng.randrange(mlo, mhi)
+        if math.gcd(m, D) != 1:
+            continue
+        out.append(m * K + rng.randrange(-xr, xr + 1) | 1)
+    return out
+
+
+def check(engine, vals, kind=pgap.FERMAT2, bases=()):
+    limbs = max(pgap.limbs_needed(v) for v in vals)
+    arr = pgap.to_limbs(vals, limbs)
+    got = engine.prp(arr, kind, bases)
+    want = ol.verdicts(arr, kind, bases)
+    bad = np.nonzero(got != want)[0]
+    assert len(bad) == 0, [(hex(vals[i]), int(got[i]), int(want[i])) for i in bad[:5]]
+    return got
+
+
+def test_special_values_fermat(engine):
+    vals = special_values()
+    for v in vals:
+        got = engine.prp_ints([v])[0]
+        assert got == ol.fermat2(v), hex(v)
+    check(engine, vals)
+
+
+def test_special_values_sprp(engine):
+    vals = special_values()
+    for bases in ((2,), (3,), (2, 3, 5), (2, 3, 5, 7)):
+        check(engine, [v for v in vals if v < (1 << 1700)], pgap.SPRP, bases)
+
+
+@pytest.mark.parametrize("bits", [30, 60, 64, 100, 200, 400, 700, 706, 900, 1108, 1120, 1300, 1336])
+def test_random_sizes(engine, bits):
+    rng = random.Random(bits)
+    vals = [rng.getrandbits(bits) | (1 << (bits - 1)) | 1 for _ in range(300)]
+    p = []
+    for v in vals[:6]:
+        x = v
+        while not ol.probab_prime(x):
+            x += 2
+        p.append(x)
+    got = check(engine, vals + p)
+    assert got[-6:].all()
+
+
+def test_config_families(engine):
+    # 503#/210 (N ~ 700 bits) and 809#/1841070 (N ~ 1108 bits), the two
+    # sizes the single-lane variants serve.
+    a = candidates(503, 210, 1, 2000, 4000, 3000, 11)
+    b = candidates(809, 1841070, 10_000_000, 12_000_000, 15000, 3000, 12)
+    ga = check(engine, a)
+    gb = check(engine, b)
+    assert ga.sum() > 0 and gb.sum() > 0
+    check(engine, a[:500], pgap.SPRP, (2,))
+    check(engine, b[:500], pgap.SPRP, (3, 5))
+
+
+def test_mixed_limb_padding(engine):
+    # the same numbers padded to a wider row give the same verdicts
+    vals = candidates(503, 210, 1, 2000, 4000, 500, 5)
+    arr = pgap.to_limbs(vals, 40)
+    assert (engine.prp(arr) == ol.verdicts(arr)).all()
+
+
+def test_launch_info(engine):
+    vals = candidates(809, 1841070, 10_000_000, 12_000_000, 15000, 1024, 3)
+    engine.prp_ints(vals)
+    info = engine.last_launch()
+    assert info.tests == 1024 and info.kernel_ms > 0 and info.radix_bits in (28, 32)
