@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X PRP engine on its headline workload.
+
+Workload (the fixed-width throughput batch): N = m*809#/1841070 + x with m
+uniform in [1e7, 1.2e7) coprime to 1841070, x uniform in [-15000, 15000],
+every N free of prime factors <= 1e6 (screened on the device), 2^20
+candidates (N ~ 1108 bits), seeded.  One step = one Fermat base-2 test of
+every candidate, inputs resident in HBM.  Reported: tests/s over exactly
+--steps steps, the dominant kernel's roofline against the int32
+multiply-add peak, and the GMP oracle timed on the host on a bounded sample
+of the same candidates (cpu_baseline).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "prime-gap_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "Fermat PRP tests/sec at 1170-bit & 2180-bit; streamed per-m p50/p99 latency"
+P, D = 809, 1841070
+M_LO, M_HI = 10_000_000, 12_000_000
+X_RANGE = 15000
+SIEVE_LIMIT = 1_000_000
+# One v_mad_u64_u32 per lane per clock: 256 CUs x 64 lanes x 2.4 GHz.
+PEAK_INT32_MAC = 256 * 64 * 2.4e9
+
+
+def draw_candidates(engine, fam, count, seed):
+    """Seeded (m, x) pairs whose N = m*K + x survive trial division to 1e6."""
+    rng = np.random.default_rng(seed)
+    ms, xs = [], []
+    have = 0
+    while have < count:
+        m = rng.integers(M_LO, M_HI, size=1 << 22, dtype=np.int64)
+        m = m[np.gcd(m, D) == 1]
+        x = rng.integers(-X_RANGE, X_RANGE + 1, size=len(m), dtype=np.int64)
+        keep = engine.filter_candidates(fam, m.astype(np.uint64), x, SIEVE_LIMIT).astype(bool)
+        ms.append(m[keep])
+        xs.append(x[keep])
+        have += int(keep.sum())
+    return np.concatenate(ms)[:count].astype(np.uint64), np.concatenate(xs)[:count]
+
+
+def load_traffic(kernel_tag):
+    """HBM bytes per dispatch measured with rocprofv3 --pmc (profiles/)."""
+    path = os.path.join(REPO, "profiles", f"{kernel_tag}.traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_dispatch")
+    except OSError:
+        return None
+
+
+def cpu_baseline(numbers_host, sample, threads):
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib  # the GMP oracle: timed as the CPU baseline, never as the product
+    arr = np.ascontiguousarray(numbers_host[:sample], dtype=np.uint32)
+    t0 = time.perf_counter()
+    verdicts = oracle_lib.verdicts(arr, 0, threads=threads)
+    dt = time.perf_counter() - t0
+    return arr.shape[0] / dt, dt, verdicts
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--count", type=int, default=1 << 20)
+    ap.add_argument("--seed", type=int, default=20261202 + 1)
+    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(m's and candidates are independent, so a multi-GPU run is a host-side partition "
+              "of this workload).", file=sys.stderr)
+        return 2
+
+    import hipmem
+    import pgap
+
+    engine = pgap.Engine(0)   # raises PgapError when no GPU / no HIP code object
+    fam = engine.family(P, D, 0)
+    t_setup = time.perf_counter()
+    m, x = draw_candidates(engine, fam, args.count, args.seed)
+    K = math.prod(p for p in range(2, P + 1) if all(p % q for q in range(2, int(p ** 0.5) + 1))) // D
+    max_n = K * M_HI + X_RANGE
+    bits_hi = max_n.bit_length()
+    limbs = (bits_hi + 31) // 32
+    d_n = hipmem.DeviceArray((args.count, limbs), np.uint32)
+    d_out = hipmem.DeviceArray(args.count, np.uint8)
+    engine.build_candidates_device(fam, m, x, limbs, d_n.ptr)
+    hipmem.synchronize()
+    # exponent bits per test: bitlen(N - 1) = bitlen(N) (N odd > 2)
+    e_bits = np.array([(int(mi) * K + int(xi)).bit_length() for mi, xi in zip(m[:4096], x[:4096])])
+    e_mean = float(e_bits.mean())
+    setup_s = time.perf_counter() - t_setup
+
+    def step():
+        engine.prp_device(d_n.ptr, limbs, args.count, d_out.ptr, pgap.FERMAT2, (), bits_hi)
+        engine.sync()
+        return engine.last_launch().kernel_ms
+
+    for _ in range(args.warmup):
+        step()
+    hipmem.synchronize()   # device-wide barrier (the process holds no torch context)
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kernel_ms.append(step())
+    hipmem.synchronize()
+    elapsed = time.perf_counter() - t0
+    info = engine.last_launch()
+
+    tests = args.count * args.steps
+    value = tests / elapsed
+    avg_kernel_s = float(np.mean(kernel_ms)) * 1e-3
+    # algorithmic work per test W = 2 L^2 e int32 MACs, L = 32-bit limbs of N
+    L32 = (int(e_mean + 0.999) + 31) // 32
+    w_test = 2 * L32 * L32 * e_mean
+    achieved = w_test * args.count / avg_kernel_s
+    w_container = 2 * 40 * 40 * e_mean   # the 1280-bit (40-limb) container
+
+    verd = d_out.to_host()
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "tests/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"fixed-width Fermat base-2 batch: N = m*{P}#/{D} + x, m~U[1e7,1.2e7) coprime to d, "
+                        f"x~U[-15000,15000], no prime factor <= 1e6, {args.count} candidates "
+                        f"({int(e_bits.min())}-{int(e_bits.max())} bits), seed {args.seed}",
+            "global_batch": args.count,
+            "bits": round(e_mean, 1),
+            "kernel": f"radix-2^{info.radix_bits} x {info.digits} digits, {info.lanes_per_number} lane(s)/number",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved / 1e12,
+            "peak": PEAK_INT32_MAC / 1e12,
+            "unit": "TOP/s (int32 multiply-add)",
+            "frac": achieved / PEAK_INT32_MAC,
+            "traffic": load_traffic(f"fermat2_r{info.radix_bits}_d{info.digits}"),
+            "work_per_test": f"2*L^2*e int32 MACs, L={L32} limbs, e={e_mean:.1f}",
+            "frac_container_40_limbs": w_container * args.count / avg_kernel_s / PEAK_INT32_MAC,
+            "kernel_ms_avg": avg_kernel_s * 1e3,
+        },
+        "prp_found": int(verd.sum()),
+        "setup_s": round(setup_s, 2),
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        host = d_n.to_host(rows=args.cpu_sample)
+        rate, dt, cv = cpu_baseline(host, args.cpu_sample, threads)
+        mism = int((cv != verd[: args.cpu_sample]).sum())
+        result["cpu_baseline"] = {
+            "value": rate,
+            "unit": "tests/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"first {args.cpu_sample} of the same candidates, Fermat base 2 via GMP mpz_powm, "
+                      f"{threads} threads, {dt:.1f} s",
+        }
+        result["parity_sample_mismatches"] = mism
+    print(json.dumps(result), flush=True)
+    engine.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

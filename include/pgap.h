@@ -106,6 +106,24 @@ pgap_status pgap_last_launch(const pgap_engine *e, pgap_launch_info *info);
  * every request name only m. */
 pgap_status pgap_family(pgap_engine *e, uint32_t P, uint32_t D, uint32_t sieve_length, int *family_id);
 
+/* Candidate construction and filtering for a family, on the device.
+ *
+ * pgap_build_candidates_device writes N_i = m_i*K + x_i as `limbs` words per
+ * row into d_out (device memory, [count][limbs]); it replaces the host-side
+ * mpz_add_ui/mpz_sub_ui + mpz_export batch fill of gap_test_gpu.cu:431-476
+ * and miller_rabin.h:352-354.  m and x are host arrays.  PGAP_E_TOO_LARGE if
+ * some N does not fit `limbs` words or is negative.
+ *
+ * pgap_filter_candidates sets keep[i] = 1 when m_i*K + x_i has no prime
+ * factor <= limit (trial division by every prime <= limit on the device, the
+ * test combined_sieve applies to each offset); used to draw sieved synthetic
+ * candidates and to screen offsets beyond the sieve length.  N itself equal
+ * to a prime <= limit is reported as 0. */
+pgap_status pgap_build_candidates_device(pgap_engine *e, int family, const uint64_t *m, const int64_t *x,
+                                         size_t count, uint32_t limbs, uint32_t *d_out);
+pgap_status pgap_filter_candidates(pgap_engine *e, int family, const uint64_t *m, const int64_t *x,
+                                   size_t count, uint32_t limit, uint8_t *keep);
+
 typedef struct {
     int family;
     uint64_t m;

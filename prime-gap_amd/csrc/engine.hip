@@ -90,7 +90,7 @@ pgap_status pgap_engine::ensure(DevBuf &b, size_t bytes, const char *what) {
     if (b.p) {
         hipError_t e = hipStreamSynchronize(stream);
         if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-        hipFree(b.p);
+        (void)hipFree(b.p);
         b.p = nullptr;
         b.cap = 0;
     }
@@ -228,15 +228,16 @@ pgap_status pgap_create(const pgap_options *opt, pgap_engine **out) {
 void pgap_destroy(pgap_engine *e) {
     if (!e) return;
     if (e->ready) {
-        hipSetDevice(e->opt.device);
-        hipStreamSynchronize(e->stream);
+        (void)hipSetDevice(e->opt.device);
+        (void)hipStreamSynchronize(e->stream);
         e->scan_shutdown();
-        for (DevBuf *b : {&e->d_n, &e->d_out, &e->d_small}) {
-            if (b->p) hipFree(b->p);
+        e->families_shutdown();
+        for (DevBuf *b : {&e->d_n, &e->d_out, &e->d_small, &e->d_m, &e->d_x, &e->d_keep}) {
+            if (b->p) (void)hipFree(b->p);
         }
-        hipEventDestroy(e->ev0);
-        hipEventDestroy(e->ev1);
-        hipStreamDestroy(e->stream);
+        (void)hipEventDestroy(e->ev0);
+        (void)hipEventDestroy(e->ev1);
+        (void)hipStreamDestroy(e->stream);
     }
     delete e;
 }
@@ -259,7 +260,7 @@ pgap_status pgap_prp_batch(pgap_engine *e, uint32_t limbs, const uint32_t *n, si
     Bases bs;
     pgap_status st = check_bases(e, kind, bases, nbases, &bs);
     if (st) return st;
-    hipSetDevice(e->opt.device);
+    (void)hipSetDevice(e->opt.device);
     const uint32_t bits = host_max_bits(n, limbs, count);
     const Variant *v;
     if ((st = e->pick((int)kind, std::max(bits, 2u), &v))) return st;
@@ -291,7 +292,7 @@ pgap_status pgap_prp_batch_device(pgap_engine *e, uint32_t limbs, const uint32_t
     Bases bs;
     pgap_status st = check_bases(e, kind, bases, nbases, &bs);
     if (st) return st;
-    hipSetDevice(e->opt.device);
+    (void)hipSetDevice(e->opt.device);
     if (max_bits == 0) {
         if ((st = e->ensure(e->d_small, 64, "scratch"))) return st;
         hipError_t he = hipMemsetAsync(e->d_small.p, 0, sizeof(uint32_t), e->stream);

@@ -9,12 +9,30 @@
 #include "../../include/pgap.h"
 #include "kernels.h"
 
+#include <vector>
+
+#include "hostbn.h"
+
 namespace pgap {
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
 };
 class Scanner;
+
+// K = P#/d with what the device needs to build and screen m*K + x.
+struct Family {
+    uint32_t P = 0, D = 0, SL = 0;
+    Words K;                 // host copy
+    double K_log = 0;        // ln K (gap_common.cpp:100-104)
+    uint32_t *d_K = nullptr; // device copy, K.size() words
+    // trial-division tables for pgap_filter_candidates (built per limit)
+    uint32_t filter_limit = 0;
+    size_t nprimes = 0;
+    uint32_t *d_primes = nullptr;
+    uint32_t *d_kmod = nullptr;  // K mod p
+    std::vector<uint32_t> small_primes, small_kmod;  // host copy for p <= 65536
+};
 }  // namespace pgap
 
 struct pgap_engine {
@@ -28,8 +46,14 @@ struct pgap_engine {
     mutable std::mutex err_mu;
     std::string err;
 
-    pgap::DevBuf d_n, d_out, d_small;
+    pgap::DevBuf d_n, d_out, d_small, d_m, d_x, d_keep;
+    std::vector<pgap::Family> families;
     pgap::Scanner *scanner = nullptr;  // owned; released by scan_shutdown()
+
+    pgap::Family *family(int id) {
+        return (id >= 0 && (size_t)id < families.size()) ? &families[(size_t)id] : nullptr;
+    }
+    void families_shutdown();
 
     void set_error(const char *fmt, ...) __attribute__((format(printf, 2, 3)));
     pgap_status hip_fail(hipError_t e, const char *what);

@@ -17,11 +17,6 @@ pgap_status pgap_prp_batch_mixed(pgap_engine *e, const uint32_t *, const uint64_
     e->set_error("mixed batches: not built yet");
     return PGAP_E_STATE;
 }
-pgap_status pgap_family(pgap_engine *e, uint32_t, uint32_t, uint32_t, int *) {
-    if (!e) return PGAP_E_ARG;
-    e->set_error("scan: not built yet");
-    return PGAP_E_STATE;
-}
 pgap_status pgap_scan_submit(pgap_engine *e, const pgap_scan_req *) {
     if (!e) return PGAP_E_ARG;
     e->set_error("scan: not built yet");

@@ -35,7 +35,7 @@ EXPORTED = (
     "pgap_create", "pgap_destroy", "pgap_last_error", "pgap_version",
     "pgap_prp_batch", "pgap_prp_batch_device", "pgap_prp_batch_mixed", "pgap_sync",
     "pgap_last_launch", "pgap_family", "pgap_scan_submit", "pgap_scan_poll",
-    "pgap_scan_in_flight", "pgap_scan_config",
+    "pgap_scan_in_flight", "pgap_scan_config", "pgap_build_candidates_device", "pgap_filter_candidates",
 )
 
 
@@ -123,6 +123,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.pgap_scan_in_flight.argtypes = [vp]
     lib.pgap_scan_in_flight.restype = sz
     lib.pgap_scan_config.argtypes = [vp, u32, u32]
+    lib.pgap_build_candidates_device.argtypes = [vp, ctypes.c_int, vp, vp, sz, u32, vp]
+    lib.pgap_filter_candidates.argtypes = [vp, ctypes.c_int, vp, vp, sz, u32, vp]
     for name in EXPORTED:
         f = getattr(lib, name)
         if name not in ("pgap_destroy", "pgap_last_error", "pgap_version", "pgap_scan_in_flight"):
@@ -240,6 +242,21 @@ class Engine:
         fid = ctypes.c_int()
         self._check(self.lib.pgap_family(self.h, P, D, sieve_length, ctypes.byref(fid)))
         return fid.value
+
+    def build_candidates_device(self, family: int, m: np.ndarray, x: np.ndarray, limbs: int, d_out: int):
+        mm = np.ascontiguousarray(m, dtype=np.uint64)
+        xx = np.ascontiguousarray(x, dtype=np.int64)
+        assert len(mm) == len(xx)
+        self._check(self.lib.pgap_build_candidates_device(self.h, family, mm.ctypes.data, xx.ctypes.data, len(mm),
+                                                          limbs, d_out))
+
+    def filter_candidates(self, family: int, m: np.ndarray, x: np.ndarray, limit: int) -> np.ndarray:
+        mm = np.ascontiguousarray(m, dtype=np.uint64)
+        xx = np.ascontiguousarray(x, dtype=np.int64)
+        keep = np.zeros(len(mm), dtype=np.uint8)
+        self._check(self.lib.pgap_filter_candidates(self.h, family, mm.ctypes.data, xx.ctypes.data, len(mm), limit,
+                                                    keep.ctypes.data))
+        return keep
 
     def scan_config(self, depth: int, confirm_rounds: int):
         self._check(self.lib.pgap_scan_config(self.h, depth, confirm_rounds))
