@@ -163,8 +163,11 @@ static pgap_status check_bases(pgap_engine *e, pgap_test kind, const uint32_t *b
     }
     out->n = nbases;
     for (uint32_t i = 0; i < nbases; ++i) {
-        if (bases[i] < 2) {
-            e->set_error("SPRP base %u < 2", bases[i]);
+        // bases are small primes in practice (generate_primes, miller_rabin.h:
+        // 269-286); the kernels multiply by the base as a scalar, which the
+        // lazy-reduction bound allows for bases below 64.
+        if (bases[i] < 2 || bases[i] > 63) {
+            e->set_error("SPRP base %u outside [2, 63]", bases[i]);
             return PGAP_E_ARG;
         }
         out->b[i] = bases[i];
@@ -232,7 +235,7 @@ void pgap_destroy(pgap_engine *e) {
         (void)hipStreamSynchronize(e->stream);
         e->scan_shutdown();
         e->families_shutdown();
-        for (DevBuf *b : {&e->d_n, &e->d_out, &e->d_small, &e->d_m, &e->d_x, &e->d_keep}) {
+        for (DevBuf *b : {&e->d_n, &e->d_out, &e->d_small, &e->d_m, &e->d_x, &e->d_keep, &e->d_idx0, &e->d_idx1}) {
             if (b->p) (void)hipFree(b->p);
         }
         (void)hipEventDestroy(e->ev0);

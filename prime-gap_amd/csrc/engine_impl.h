@@ -31,7 +31,8 @@ struct Family {
     size_t nprimes = 0;
     uint32_t *d_primes = nullptr;
     uint32_t *d_kmod = nullptr;  // K mod p
-    std::vector<uint32_t> small_primes, small_kmod;  // host copy for p <= 65536
+    uint64_t *d_mu = nullptr;    // floor((2^64-1)/p) for Barrett reduction
+    uint32_t stage_end[2] = {0, 0};  // prime-index ends of the first two filter stages
 };
 }  // namespace pgap
 
@@ -46,7 +47,7 @@ struct pgap_engine {
     mutable std::mutex err_mu;
     std::string err;
 
-    pgap::DevBuf d_n, d_out, d_small, d_m, d_x, d_keep;
+    pgap::DevBuf d_n, d_out, d_small, d_m, d_x, d_keep, d_idx0, d_idx1;
     std::vector<pgap::Family> families;
     pgap::Scanner *scanner = nullptr;  // owned; released by scan_shutdown()
 

@@ -23,27 +23,44 @@ __global__ void kmod_kernel(const uint32_t *__restrict__ K, uint32_t nk, const u
     out[i] = (uint32_t)r;
 }
 
-// keep[i] = 1 iff m_i*K + x_i has no prime factor in the table.
-__global__ void filter_kernel(const uint64_t *__restrict__ m, const int64_t *__restrict__ x, uint32_t count,
-                              const uint32_t *__restrict__ primes, const uint32_t *__restrict__ kmod, uint32_t np,
-                              uint8_t *__restrict__ keep) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= count) return;
+// a mod p with mu = floor((2^64-1)/p): the quotient estimate is at most 2
+// short, so two conditional subtractions finish it.
+__device__ __forceinline__ uint64_t mod_barrett(uint64_t a, uint64_t p, uint64_t mu) {
+    const uint64_t q = __umul64hi(a, mu);
+    uint64_t r = a - q * p;
+    r = r >= p ? r - p : r;
+    return r >= p ? r - p : r;
+}
+
+// One stage of the trial-division screen over primes [k0, k1).  Stage 0
+// reads every candidate; later stages read the survivors the previous stage
+// appended to idx_in (count in *n_in), so waves stay dense instead of
+// idling behind the few lanes that survive the small primes.
+__global__ void filter_stage_kernel(const uint64_t *__restrict__ m, const int64_t *__restrict__ x,
+                                    const uint32_t *__restrict__ idx_in, const uint32_t *__restrict__ n_in,
+                                    uint32_t count, const uint32_t *__restrict__ primes,
+                                    const uint32_t *__restrict__ kmod, const uint64_t *__restrict__ mu, uint32_t k0,
+                                    uint32_t k1, int last, uint8_t *__restrict__ keep, uint32_t *__restrict__ idx_out,
+                                    uint32_t *__restrict__ n_out) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t n = idx_in ? *n_in : count;
+    if (t >= n) return;
+    const uint32_t i = idx_in ? idx_in[t] : t;
     const uint64_t mi = m[i];
     const int64_t xi = x[i];
-    const uint64_t ax = (uint64_t)(xi < 0 ? -xi : xi);
-    uint8_t ok = 1;
-    for (uint32_t k = 0; k < np; ++k) {
-        const uint64_t p = primes[k];
-        const uint64_t mr = (mi % p) * kmod[k] % p;   // m*K mod p
-        const uint64_t xr = ax % p;
-        const uint64_t r = xi < 0 ? (mr + p - xr) % p : (mr + xr) % p;
-        if (r == 0) {
-            ok = 0;
-            break;
-        }
+    const uint64_t ax = xi < 0 ? (uint64_t)(-(xi + 1)) + 1u : (uint64_t)xi;
+    for (uint32_t k = k0; k < k1; ++k) {
+        const uint64_t p = primes[k], u = mu[k];
+        const uint64_t mr = mod_barrett(mod_barrett(mi, p, u) * kmod[k], p, u);  // m*K mod p
+        const uint64_t xr = mod_barrett(ax, p, u);
+        const bool hit = xi < 0 ? (mr == xr) : ((mr + xr == p) || (mr == 0 && xr == 0));
+        if (hit) return;  // keep[i] stays 0
     }
-    keep[i] = ok;
+    if (last) {
+        keep[i] = 1;
+    } else {
+        idx_out[atomicAdd(n_out, 1u)] = i;
+    }
 }
 
 // out[i] = m_i*K + x_i as `limbs` words; *err |= 1 on overflow or negative.
@@ -101,33 +118,45 @@ __global__ void build_kernel(const uint32_t *__restrict__ K, uint32_t nk, const 
 
 using namespace pgap;
 
+static void free_tables(Family &f) {
+    if (f.d_primes) (void)hipFree(f.d_primes);
+    if (f.d_kmod) (void)hipFree(f.d_kmod);
+    if (f.d_mu) (void)hipFree(f.d_mu);
+    f.d_primes = f.d_kmod = nullptr;
+    f.d_mu = nullptr;
+    f.filter_limit = 0;
+}
+
 void pgap_engine::families_shutdown() {
     for (Family &f : families) {
         if (f.d_K) (void)hipFree(f.d_K);
-        if (f.d_primes) (void)hipFree(f.d_primes);
-        if (f.d_kmod) (void)hipFree(f.d_kmod);
         f.d_K = nullptr;
-        f.d_primes = f.d_kmod = nullptr;
+        free_tables(f);
     }
     families.clear();
 }
 
 static pgap_status family_tables(pgap_engine *e, Family &f, uint32_t limit) {
     if (f.filter_limit == limit && f.d_primes) return PGAP_OK;
+    free_tables(f);
     std::vector<uint32_t> ps = primes_upto(limit);
-    if (f.d_primes) (void)hipFree(f.d_primes);
-    if (f.d_kmod) (void)hipFree(f.d_kmod);
-    f.d_primes = f.d_kmod = nullptr;
-    f.filter_limit = 0;
+    std::vector<uint64_t> mu(ps.size());
+    for (size_t i = 0; i < ps.size(); ++i) mu[i] = UINT64_MAX / ps[i];
+    f.stage_end[0] = (uint32_t)(std::lower_bound(ps.begin(), ps.end(), 1u << 10) - ps.begin());
+    f.stage_end[1] = (uint32_t)(std::lower_bound(ps.begin(), ps.end(), 1u << 15) - ps.begin());
     const size_t bytes = std::max<size_t>(ps.size(), 1) * sizeof(uint32_t);
     hipError_t he;
-    if ((he = hipMalloc(&f.d_primes, bytes)) != hipSuccess || (he = hipMalloc(&f.d_kmod, bytes)) != hipSuccess) {
+    if ((he = hipMalloc(&f.d_primes, bytes)) != hipSuccess || (he = hipMalloc(&f.d_kmod, bytes)) != hipSuccess ||
+        (he = hipMalloc(&f.d_mu, 2 * bytes)) != hipSuccess) {
         (void)hipGetLastError();
         e->set_error("hipMalloc for %zu filter primes: %s", ps.size(), hipGetErrorString(he));
+        free_tables(f);
         return PGAP_E_OOM;
     }
     if (!ps.empty()) {
         he = hipMemcpyAsync(f.d_primes, ps.data(), ps.size() * sizeof(uint32_t), hipMemcpyHostToDevice, e->stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(f.d_mu, mu.data(), mu.size() * sizeof(uint64_t), hipMemcpyHostToDevice, e->stream);
         if (he != hipSuccess) return e->hip_fail(he, "upload primes");
         hipLaunchKernelGGL(kmod_kernel, dim3((uint32_t)((ps.size() + 255) / 256)), dim3(256), 0, e->stream, f.d_K,
                            (uint32_t)f.K.size(), f.d_primes, (uint32_t)ps.size(), f.d_kmod);
@@ -230,16 +259,33 @@ pgap_status pgap_filter_candidates(pgap_engine *e, int family, const uint64_t *m
     pgap_status st;
     if ((st = family_tables(e, *f, limit))) return st;
     if ((st = e->ensure(e->d_m, count * 8, "m")) || (st = e->ensure(e->d_x, count * 8, "x")) ||
-        (st = e->ensure(e->d_keep, count, "keep")))
+        (st = e->ensure(e->d_keep, count, "keep")) || (st = e->ensure(e->d_idx0, count * 4, "index")) ||
+        (st = e->ensure(e->d_idx1, count * 4, "index")) || (st = e->ensure(e->d_small, 64, "counters")))
         return st;
     hipError_t he;
     if ((he = hipMemcpyAsync(e->d_m.p, m, count * 8, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
-        (he = hipMemcpyAsync(e->d_x.p, x, count * 8, hipMemcpyHostToDevice, e->stream)) != hipSuccess)
+        (he = hipMemcpyAsync(e->d_x.p, x, count * 8, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
+        (he = hipMemsetAsync(e->d_keep.p, 0, count, e->stream)) != hipSuccess ||
+        (he = hipMemsetAsync(e->d_small.p, 0, 16, e->stream)) != hipSuccess)
         return e->hip_fail(he, "filter upload");
-    hipLaunchKernelGGL(filter_kernel, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0, e->stream,
-                       (const uint64_t *)e->d_m.p, (const int64_t *)e->d_x.p, (uint32_t)count, f->d_primes, f->d_kmod,
-                       (uint32_t)f->nprimes, (uint8_t *)e->d_keep.p);
-    if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "filter kernel");
+    // three stages: primes < 2^10, < 2^15, the rest; survivors are compacted
+    // between stages (counters in d_small[0], d_small[1]).
+    const uint32_t ends[3] = {std::min(f->stage_end[0], (uint32_t)f->nprimes),
+                              std::min(f->stage_end[1], (uint32_t)f->nprimes), (uint32_t)f->nprimes};
+    uint32_t *ctr = (uint32_t *)e->d_small.p;
+    uint32_t *idx[2] = {(uint32_t *)e->d_idx0.p, (uint32_t *)e->d_idx1.p};
+    uint32_t k0 = 0;
+    for (int s = 0; s < 3; ++s) {
+        const bool last = (s == 2);
+        const uint32_t *in = s == 0 ? nullptr : idx[(s - 1) & 1];
+        const uint32_t *n_in = s == 0 ? nullptr : ctr + (s - 1);
+        hipLaunchKernelGGL(filter_stage_kernel, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0, e->stream,
+                           (const uint64_t *)e->d_m.p, (const int64_t *)e->d_x.p, in, n_in, (uint32_t)count,
+                           f->d_primes, f->d_kmod, f->d_mu, k0, ends[s], last ? 1 : 0, (uint8_t *)e->d_keep.p,
+                           last ? nullptr : idx[s & 1], last ? nullptr : ctr + s);
+        if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "filter kernel");
+        k0 = ends[s];
+    }
     if ((he = hipMemcpyAsync(keep, e->d_keep.p, count, hipMemcpyDeviceToHost, e->stream)) != hipSuccess ||
         (he = hipStreamSynchronize(e->stream)) != hipSuccess)
         return e->hip_fail(he, "filter sync");

@@ -155,19 +155,23 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&a)[L], const uint32_t (&n)[L
     a[L - 1] = (uint32_t)T & MASK;
 }
 
-// c <- a * b * R^{-1} mod N, lazy (a, b < 4N, N < R/16 => c < 2N).
+// Wide variant (L up to 127): the 2^s factor is applied to the finished
+// digits (a one-bit shift with carry) instead of to the column sums, whose
+// extra bit would break the 64-bit column bound above 63 digits.
 template <int L>
-__device__ __forceinline__ void mul_redc(uint32_t (&c)[L], const uint32_t (&a)[L], const uint32_t (&b)[L],
-                                         const uint32_t (&n)[L], uint32_t np) {
+__device__ __forceinline__ void sqr_redc_wide(uint32_t (&a)[L], const uint32_t (&n)[L], uint32_t np,
+                                              uint32_t s) {
+    static_assert(L >= 2 && L <= 127, "radix-2^28 column bound");
     uint32_t m[L];
-    uint32_t r[L];
     uint64_t T = 0;
 #pragma unroll
     for (int k = 0; k < 2 * L - 1; ++k) {
         const int ilo = k - (L - 1) > 0 ? k - (L - 1) : 0;
-        const int ihi = k < L - 1 ? k : L - 1;
+        uint64_t X = 0;
 #pragma unroll
-        for (int i = ilo; i <= ihi; ++i) T = mad(a[i], b[k - i], T);
+        for (int i = ilo; 2 * i < k; ++i) X = mad(a[i], a[k - i], X);
+        T = (X << 1) + T;
+        if ((k & 1) == 0) T = mad(a[k >> 1], a[k >> 1], T);
         if (k < L) {
 #pragma unroll
             for (int i = ilo; i < k; ++i) T = mad(m[i], n[k - i], T);
@@ -177,13 +181,19 @@ __device__ __forceinline__ void mul_redc(uint32_t (&c)[L], const uint32_t (&a)[L
         } else {
 #pragma unroll
             for (int i = ilo; i < L; ++i) T = mad(m[i], n[k - i], T);
-            r[k - L] = (uint32_t)T & MASK;
+            a[k - L] = (uint32_t)T & MASK;
             T >>= 28;
         }
     }
-    r[L - 1] = (uint32_t)T & MASK;
+    a[L - 1] = (uint32_t)T & MASK;
+    // a <- a * 2^s (a < 2N < R/8, so the shifted value still fits L digits)
+    uint32_t c = 0;
 #pragma unroll
-    for (int j = 0; j < L; ++j) c[j] = r[j];
+    for (int j = 0; j < L; ++j) {
+        const uint32_t v = (a[j] << s) | c;
+        c = v >> 28;
+        a[j] = v & MASK;
+    }
 }
 
 }  // namespace r28

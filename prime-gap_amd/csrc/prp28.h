@@ -40,6 +40,31 @@ __device__ __forceinline__ void mont_one_two(const uint32_t (&n)[L], int b, uint
     mod_double<L>(two, n);
 }
 
+// One exponent step x <- x^2 R^{-1} 2^s: the column-folded doubling up to 62
+// digits, the digit-pass doubling beyond (the column bound of the folded form).
+template <int L>
+__device__ __forceinline__ void sqr_step(uint32_t (&x)[L], const uint32_t (&n)[L], uint32_t np, uint32_t s) {
+    if constexpr (L <= 62)
+        sqr_redc<L>(x, n, np, s);
+    else
+        sqr_redc_wide<L>(x, n, np, s);
+}
+
+// x <- x * b for a small base b (b < 64): digits times b with the carry
+// propagated.  Lazy bound: x < 2N in gives x < 2bN out, and a squaring of an
+// input < 2bN returns < 2N as long as N < R / (4 b^2), which the strong-PRP
+// variants' max_bits (28L - 14) guarantees for every b < 64.
+template <int L>
+__device__ __forceinline__ void mul_small(uint32_t (&x)[L], uint32_t b) {
+    uint64_t c = 0;
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        const uint64_t v = mad(x[j], b, c);  // < 2^34
+        c = v >> 28;
+        x[j] = (uint32_t)v & MASK;
+    }
+}
+
 // Fermat base 2.  The top bit of E = N-1 is consumed by starting at Mont(2);
 // every further bit is one squaring with the doubling folded in.
 template <int L>
@@ -59,7 +84,7 @@ __device__ __forceinline__ uint32_t fermat2(const uint32_t* __restrict__ nw, uin
         const uint32_t s = (i == 0) ? 0u : ((ew >> (i & 31)) & 1u);
         uint32_t nxt = ew;
         if ((i & 31) == 0 && i > 0) nxt = nw[(i >> 5) - 1];
-        sqr_redc<L>(x, n, np, s);
+        sqr_step<L>(x, n, np, s);
         ew = nxt;
         --i;
     }
@@ -80,69 +105,36 @@ __device__ __forceinline__ void mod_add(uint32_t (&a)[L], const uint32_t (&b)[L]
     cond_sub<L>(a, n);
 }
 
-// Strong probable prime test to one base.
+// Strong probable prime test to one base b < 64 (N - 1 = d * 2^tz):
+// x = b^d, pass if x = +-1 or x^(2^r) = -1 for some 0 < r < tz.  One loop
+// with a single squaring call site covers both the exponent bits of d and
+// the tz - 1 checking squarings, so the wide squaring is inlined once.
 template <int L>
 __device__ __forceinline__ uint32_t sprp(const uint32_t* __restrict__ nw, uint32_t limbs, uint32_t base) {
     uint32_t n[L];
     load_n<L>(nw, limbs, n);
     if ((n[0] & 1u) == 0) return 0;
     const int b = bitlen<L>(n);
-    if (b < 2 || (b == 2 && n[0] < 3)) return 0;
+    if (b < 2) return 0;  // N = 1
     const uint32_t np = neg_inv28(n[0]);
-    uint32_t one[L], two[L];
-    mont_one_two<L>(n, b, one, two);
+    uint32_t one[L], x[L];
+    mont_one_two<L>(n, b, one, x);  // x = Mont(2)
 
-    // N - 1 = d * 2^tz
     int tz = 1;
     while (((nw[tz >> 5] >> (tz & 31)) & 1u) == 0) ++tz;
 
-    // Reduce the base below N when N fits one word.
     uint32_t bb = base;
-    if (b <= 32) {
-        const uint32_t n32 = nw[0];
-        bb = base % n32;
-    }
-    uint32_t x[L];
-    if (bb == 2) {
-#pragma unroll
-        for (int j = 0; j < L; ++j) x[j] = two[j];
-    } else {
+    if (b <= 32) bb = base % nw[0];  // N < 2^32: reduce the base
+    if (bb == 0) return 0;           // x = 0 is never +-1
+    if (bb != 2) {
         // Mont(bb) = bb * R mod N by double-and-add over the bits of bb.
 #pragma unroll
         for (int j = 0; j < L; ++j) x[j] = 0;
-        for (int t = 31; t >= 0; --t) {
+        for (int t = 5; t >= 0; --t) {
             mod_double<L>(x, n);
             if ((bb >> t) & 1u) mod_add<L>(x, one, n);
         }
     }
-    if (bb == 0) {
-        return 0;  // base == 0 mod N: x stays 0, never 1 or N-1
-    }
-    uint32_t bm[L];
-#pragma unroll
-    for (int j = 0; j < L; ++j) bm[j] = x[j];
-
-    // Left-to-right over the bits of d = (N-1) >> tz, top bit consumed.
-    int i = b - 2;
-    uint32_t ew = nw[i >> 5];
-    while (i >= tz) {
-        const uint32_t bit = (ew >> (i & 31)) & 1u;
-        uint32_t nxt = ew;
-        if ((i & 31) == 0 && i > 0) nxt = nw[(i >> 5) - 1];
-        if (bb == 2) {
-            sqr_redc<L>(x, n, np, bit);
-        } else {
-            sqr_redc<L>(x, n, np, 0u);
-            if (bit) mul_redc<L>(x, x, bm, n, np);
-        }
-        ew = nxt;
-        --i;
-    }
-    // x = Mont(base^d) < 4N (the last bit-folded squaring may double):
-    // three conditional subtractions make it canonical.
-    cond_sub<L>(x, n);
-    cond_sub<L>(x, n);
-    cond_sub<L>(x, n);
     uint32_t mone[L];  // Mont(-1) = N - Mont(1)
     {
         uint32_t br = 0;
@@ -153,14 +145,46 @@ __device__ __forceinline__ uint32_t sprp(const uint32_t* __restrict__ nw, uint32
             mone[j] = v & MASK;
         }
     }
-    if (equal<L>(x, one) || equal<L>(x, mone)) return 1;
-    for (int r = 1; r < tz; ++r) {
-        sqr_redc<L>(x, n, np, 0u);
-        cond_sub<L>(x, n);
-        if (equal<L>(x, mone)) return 1;
-        if (equal<L>(x, one)) return 0;
+
+    // Exponent bits of d below its top bit: positions b-2 .. tz of N - 1.
+    int i = b - 2;
+    int checks = 0;  // checking squarings done
+    uint32_t ew = i >= tz ? nw[i >> 5] : 0u;
+    bool have_base_power = (i < tz);  // x already equals Mont(b^d)
+    for (;;) {
+        uint32_t s = 0, bit = 0, nxt = ew;
+        const bool checking = have_base_power;
+        if (checking) {
+            // x < 2*bb*N here: bring it below N, then compare.
+            for (int r = 0; r < 128; ++r) {
+                uint32_t br = 0, u[L];
+#pragma unroll
+                for (int j = 0; j < L; ++j) {
+                    uint32_t v = x[j] - n[j] - br;
+                    br = v >> 31;
+                    u[j] = v & MASK;
+                }
+                if (br) break;
+#pragma unroll
+                for (int j = 0; j < L; ++j) x[j] = u[j];
+            }
+            if (equal<L>(x, mone)) return 1;
+            if (equal<L>(x, one)) return checks == 0 ? 1u : 0u;
+            if (checks + 1 >= tz) return 0;
+            ++checks;
+        } else {
+            bit = (ew >> (i & 31)) & 1u;
+            if ((i & 31) == 0 && i > tz) nxt = nw[(i >> 5) - 1];
+            s = bb == 2 ? bit : 0u;
+        }
+        sqr_step<L>(x, n, np, s);  // the only squaring call site
+        if (!checking) {
+            if (bb != 2 && bit) mul_small<L>(x, bb);
+            ew = nxt;
+            --i;
+            have_base_power = (i < tz);
+        }
     }
-    return 0;
 }
 
 }  // namespace r28

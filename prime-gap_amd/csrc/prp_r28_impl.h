@@ -39,7 +39,7 @@ template <int... Ls>
 struct RegisterR28 {
     RegisterR28() {
         (register_variant(Variant{28, Ls, 1, KIND_FERMAT2, 28 * Ls - 4, &launch_r28<Ls, KIND_FERMAT2>}), ...);
-        (register_variant(Variant{28, Ls, 1, KIND_SPRP, 28 * Ls - 4, &launch_r28<Ls, KIND_SPRP>}), ...);
+        (register_variant(Variant{28, Ls, 1, KIND_SPRP, 28 * Ls - 14, &launch_r28<Ls, KIND_SPRP>}), ...);
     }
 };
 

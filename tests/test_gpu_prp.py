@@ -56,7 +56,8 @@ def test_special_values_sprp(engine):
         check(engine, [v for v in vals if v < (1 << 1700)], pgap.SPRP, bases)
 
 
-@pytest.mark.parametrize("bits", [30, 60, 64, 100, 200, 400, 700, 706, 900, 1108, 1120, 1300, 1336])
+@pytest.mark.parametrize("bits", [30, 60, 64, 100, 200, 400, 700, 706, 900, 1108, 1120, 1300, 1336, 1400,
+                                  1650, 1730, 1750, 1900, 2000, 2129, 2180, 2226])
 def test_random_sizes(engine, bits):
     rng = random.Random(bits)
     vals = [rng.getrandbits(bits) | (1 << (bits - 1)) | 1 for _ in range(300)]
@@ -80,6 +81,26 @@ def test_config_families(engine):
     assert ga.sum() > 0 and gb.sum() > 0
     check(engine, a[:500], pgap.SPRP, (2,))
     check(engine, b[:500], pgap.SPRP, (3, 5))
+
+
+def test_large_family_and_sprp(engine):
+    # 1511#/2190 (N ~ 2109-2129 bits): the wide one-lane variants
+    c = candidates(1511, 2190, 1, 1_000_000, 20000, 600, 13)
+    assert max(v.bit_length() for v in c) > 2100
+    assert check(engine, c).sum() >= 0
+    check(engine, c[:200], pgap.SPRP, (3, 5))
+    check(engine, c[:100], pgap.SPRP, (2, 61))
+    m2203 = (1 << 2203) - 1
+    assert engine.prp_ints([m2203])[0] == 1
+
+
+def test_rejects_what_no_variant_covers(engine):
+    with pytest.raises(pgap.PgapError) as ei:
+        engine.prp_ints([(1 << 9000) + 1])
+    assert ei.value.status == pgap.PGAP_E_TOO_LARGE
+    with pytest.raises(pgap.PgapError) as ei:
+        engine.prp_ints([101], pgap.SPRP, (64,))
+    assert ei.value.status == pgap.PGAP_E_ARG
 
 
 def test_mixed_limb_padding(engine):
