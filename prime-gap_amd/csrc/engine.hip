@@ -125,7 +125,7 @@ pgap_status pgap_engine::launch_batch(const Variant *v, const uint32_t *d_n, uin
     uint32_t launches = 0;
     for (size_t off = 0; off < count; off += chunk) {
         const uint32_t c = (uint32_t)std::min(chunk, count - off);
-        e = v->launch(d_n + off * limbs, limbs, c, bases, d_out + off, opt.waves_per_simd, stream);
+        e = v->launch(d_n + off * limbs, limbs, c, nullptr, nullptr, bases, d_out + off, opt.waves_per_simd, stream);
         if (e != hipSuccess) return hip_fail(e, "kernel launch");
         ++launches;
     }

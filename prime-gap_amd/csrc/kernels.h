@@ -19,9 +19,14 @@ struct Bases {
 
 enum Kind { KIND_FERMAT2 = 0, KIND_SPRP = 1 };
 
-// Batch launch: count numbers, row-major [count][limbs] words, verdicts out.
-typedef hipError_t (*batch_launch_fn)(const uint32_t *d_n, uint32_t limbs, uint32_t count, Bases bases,
-                                      uint8_t *d_out, int waves_per_simd, hipStream_t stream);
+// Batch launch: numbers are rows of `limbs` words; verdicts out.
+//   count    grid size (upper bound of the work)
+//   d_count  if non-null, the actual count is read from device memory (so a
+//            round can size its work on the device without a host round trip)
+//   d_idx    if non-null, work item i tests row d_idx[i] and writes out[d_idx[i]]
+typedef hipError_t (*batch_launch_fn)(const uint32_t *d_n, uint32_t limbs, uint32_t count, const uint32_t *d_count,
+                                      const uint32_t *d_idx, Bases bases, uint8_t *d_out, int waves_per_simd,
+                                      hipStream_t stream);
 
 struct Variant {
     int radix;       // 28 or 32

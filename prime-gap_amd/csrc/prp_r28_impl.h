@@ -11,11 +11,14 @@ namespace pgap {
 // (row-major per number) does not bound the kernel: it is VALU-bound.
 template <int L, int KIND>
 __global__ void __launch_bounds__(256) prp_r28_kernel(const uint32_t *__restrict__ n, uint32_t limbs,
-                                                      uint32_t count, Bases bases,
+                                                      uint32_t count, const uint32_t *__restrict__ d_count,
+                                                      const uint32_t *__restrict__ d_idx, Bases bases,
                                                       uint8_t *__restrict__ out) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= count) return;
-    const uint32_t *nw = n + (size_t)i * limbs;
+    const uint32_t cnt = d_count ? min(*d_count, count) : count;
+    if (i >= cnt) return;
+    const uint32_t row = d_idx ? d_idx[i] : i;
+    const uint32_t *nw = n + (size_t)row * limbs;
     uint32_t v;
     if (KIND == KIND_FERMAT2) {
         v = r28::fermat2<L>(nw, limbs);
@@ -23,15 +26,17 @@ __global__ void __launch_bounds__(256) prp_r28_kernel(const uint32_t *__restrict
         v = 1;
         for (uint32_t k = 0; k < bases.n && v; ++k) v = r28::sprp<L>(nw, limbs, bases.b[k]);
     }
-    out[i] = (uint8_t)v;
+    out[row] = (uint8_t)v;
 }
 
 template <int L, int KIND>
-hipError_t launch_r28(const uint32_t *d_n, uint32_t limbs, uint32_t count, Bases bases, uint8_t *d_out,
-                      int /*waves_per_simd*/, hipStream_t stream) {
+hipError_t launch_r28(const uint32_t *d_n, uint32_t limbs, uint32_t count, const uint32_t *d_count,
+                      const uint32_t *d_idx, Bases bases, uint8_t *d_out, int /*waves_per_simd*/,
+                      hipStream_t stream) {
     if (count == 0) return hipSuccess;
     const dim3 grid((count + 255u) / 256u);
-    hipLaunchKernelGGL((prp_r28_kernel<L, KIND>), grid, dim3(256), 0, stream, d_n, limbs, count, bases, d_out);
+    hipLaunchKernelGGL((prp_r28_kernel<L, KIND>), grid, dim3(256), 0, stream, d_n, limbs, count, d_count, d_idx,
+                       bases, d_out);
     return hipGetLastError();
 }
 

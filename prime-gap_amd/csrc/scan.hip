@@ -1,36 +1,741 @@
-// scan.hip -- streaming scan (filled in below) and mixed-width batches.
+// scan.hip -- streaming prev/next-prime scan: device-resident per-m state,
+// continuous batching and HBM-budget admission.
+//
+// Reference counterparts: the loader/scan state machine load_batch_thread
+// (gap_test_gpu.cu:323-655) with its DataM / GPUBatch records
+// (gap_test_gpu.cu:139-215), the GPU batch thread run_gpu_thread (:232-268),
+// the beyond-SL fallback run_overflow_thread (:270-321), and the per-m scan
+// semantics of test_interval_cpu (gap_test_common.cpp:128-181).
+//
+// Design.  Every admitted m owns a slot: its centre m*K, its prev and next
+// unknown lists (outward order) and per-side state (next index to test,
+// index of the first confirmed PRP) stay in HBM until the m finishes.  A
+// round tests, for every unfinished side, the next `depth` unknowns at once:
+//   issue -> materialise N = centre -+ x -> Fermat base 2 -> collect hits ->
+//   strong-PRP confirmation of the hits -> first confirmed hit per side ->
+//   per-slot update, emitting finished m's and exhausted sides.
+// Every index below a side's first PRP has been tested in this or an earlier
+// round, so "first confirmed PRP" is exactly the reference's first
+// mpz_probab_prime_p hit; tests issued past it in the same round are
+// speculative and are not counted (p_tests / n_tests count up to the hit).
+// A side whose unknowns are exhausted continues past SL with candidates that
+// have no factor below 2^14 -- the prevprime(centre - SL) / nextprime(centre
+// + SL) fallback of gap_test_common.cpp:160-178, here on the GPU.
+// New requests are admitted between rounds as slots and budget allow, so m's
+// that arrive while others are on the device join the next round.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <vector>
+
 #include "engine_impl.h"
 
 namespace pgap {
-class Scanner {};
+
+constexpr uint32_t NONE = 0xffffffffu;
+constexpr uint32_t FALLBACK_CHUNK = 4096;  // candidates appended per exhausted side
+constexpr uint32_t FALLBACK_SPAN = 65536;  // offsets screened per append
+constexpr uint32_t MAX_SLOTS = 1u << 16;
+
+enum : uint32_t {
+    F_PREV_DONE = 1,
+    F_NEXT_DONE = 2,
+    F_PREV_WAIT = 4,  // exhausted, waiting for the host to append candidates
+    F_NEXT_WAIT = 8,
+};
+
+struct SlotState {
+    uint32_t cursor[2];
+    uint32_t found[2];
+    uint32_t len[2];
+    uint32_t flags;
+    uint32_t cwords;
+};
+
+// Event record written for a slot that finished or needs fallback candidates.
+struct Event {
+    uint32_t slot, code;  // code bit 0: finished; bit 1/2: prev/next exhausted
+    uint32_t found[2];
+    int32_t x[2];
+    uint32_t pad[2];
+};
+
+// Admission / append descriptors (host -> device, one per request).
+struct Upload {
+    uint32_t slot;
+    uint32_t kind;        // 0 = new m, 1 = append to side `side`
+    uint32_t side;
+    uint32_t cwords;
+    uint32_t n0, n1;      // list entries (kind 0: prev, next; kind 1: appended)
+    uint32_t off_center;  // word offsets into the data part
+    uint32_t off_l0, off_l1;
+    uint32_t pad[3];
+};
+
+// ---- kernels --------------------------------------------------------------
+
+__global__ void scatter_kernel(const Upload *__restrict__ up, const uint32_t *__restrict__ data, uint32_t nup,
+                               SlotState *__restrict__ st, uint32_t *__restrict__ center, uint32_t cw,
+                               int32_t *__restrict__ lists, uint32_t cap) {
+    const uint32_t u = blockIdx.x;
+    if (u >= nup) return;
+    const Upload d = up[u];
+    int32_t *l0 = lists + (size_t)d.slot * 2 * cap;
+    int32_t *l1 = l0 + cap;
+    if (d.kind == 0) {
+        uint32_t *c = center + (size_t)d.slot * cw;
+        for (uint32_t j = threadIdx.x; j < cw; j += blockDim.x) c[j] = j < d.cwords ? data[d.off_center + j] : 0u;
+        for (uint32_t j = threadIdx.x; j < d.n0; j += blockDim.x) l0[j] = (int32_t)data[d.off_l0 + j];
+        for (uint32_t j = threadIdx.x; j < d.n1; j += blockDim.x) l1[j] = (int32_t)data[d.off_l1 + j];
+        if (threadIdx.x == 0) {
+            SlotState s;
+            s.cursor[0] = s.cursor[1] = 0;
+            s.found[0] = s.found[1] = NONE;
+            s.len[0] = d.n0;
+            s.len[1] = d.n1;
+            s.flags = 0;
+            s.cwords = d.cwords;
+            st[d.slot] = s;
+        }
+    } else {
+        int32_t *l = d.side ? l1 : l0;
+        const uint32_t base = st[d.slot].len[d.side];  // read before the barrier below
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < d.n0; j += blockDim.x) l[base + j] = (int32_t)data[d.off_l0 + j];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            st[d.slot].len[d.side] = base + d.n0;
+            st[d.slot].flags &= ~(d.side ? F_NEXT_WAIT : F_PREV_WAIT);
+        }
+    }
+}
+
+// One thread per (active slot, side): reserve up to `depth` items.
+__global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, SlotState *__restrict__ st,
+                             uint32_t depth, uint32_t *__restrict__ items, uint32_t *__restrict__ item_idx,
+                             uint32_t *__restrict__ ctr) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= 2 * na) return;
+    const uint32_t slot = active[t >> 1], side = t & 1u;
+    SlotState &s = st[slot];
+    const uint32_t done = side ? F_NEXT_DONE : F_PREV_DONE;
+    const uint32_t wait = side ? F_NEXT_WAIT : F_PREV_WAIT;
+    if (s.flags & (done | wait)) return;
+    const uint32_t c = s.cursor[side], l = s.len[side];
+    if (c >= l) return;
+    const uint32_t k = min(depth, l - c);
+    const uint32_t base = atomicAdd(&ctr[0], k);
+    for (uint32_t j = 0; j < k; ++j) {
+        items[base + j] = (slot << 1) | side;
+        item_idx[base + j] = c + j;
+    }
+    s.cursor[side] = c + k;
+}
+
+// N = centre -+ x for every item, `w` words per row.
+__global__ void materialize_kernel(const uint32_t *__restrict__ items, const uint32_t *__restrict__ item_idx,
+                                   const uint32_t *__restrict__ ctr, const uint32_t *__restrict__ center,
+                                   uint32_t cw, const int32_t *__restrict__ lists, uint32_t cap,
+                                   uint32_t *__restrict__ nbuf, uint32_t w) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= ctr[0]) return;
+    const uint32_t slot = items[t] >> 1, side = items[t] & 1u;
+    const uint32_t x = (uint32_t)lists[((size_t)slot * 2 + side) * cap + item_idx[t]];
+    const uint32_t *c = center + (size_t)slot * cw;
+    uint32_t *o = nbuf + (size_t)t * w;
+    if (side) {
+        uint64_t carry = x;
+        for (uint32_t j = 0; j < w; ++j) {
+            const uint64_t v = (uint64_t)(j < cw ? c[j] : 0u) + carry;
+            o[j] = (uint32_t)v;
+            carry = v >> 32;
+        }
+    } else {
+        uint32_t borrow = x;
+        for (uint32_t j = 0; j < w; ++j) {
+            const uint32_t cj = j < cw ? c[j] : 0u;
+            o[j] = cj - borrow;
+            borrow = cj < borrow ? 1u : 0u;
+        }
+    }
+}
+
+__global__ void hits_kernel(const uint8_t *__restrict__ verdict, uint32_t *__restrict__ ctr,
+                            uint32_t *__restrict__ hits) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= ctr[0]) return;
+    if (verdict[t]) hits[atomicAdd(&ctr[1], 1u)] = t;
+}
+
+__global__ void retire_kernel(const uint32_t *__restrict__ items, const uint32_t *__restrict__ item_idx,
+                              const uint8_t *__restrict__ verdict, const uint32_t *__restrict__ ctr,
+                              SlotState *__restrict__ st) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= ctr[0] || !verdict[t]) return;
+    const uint32_t slot = items[t] >> 1, side = items[t] & 1u;
+    atomicMin(&st[slot].found[side], item_idx[t]);
+}
+
+__global__ void update_kernel(const uint32_t *__restrict__ active, uint32_t na, SlotState *__restrict__ st,
+                              const int32_t *__restrict__ lists, uint32_t cap, uint32_t *__restrict__ ctr,
+                              Event *__restrict__ events) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= na) return;
+    const uint32_t slot = active[t];
+    SlotState &s = st[slot];
+    uint32_t flags = s.flags, code = 0;
+    for (uint32_t side = 0; side < 2; ++side) {
+        const uint32_t done = side ? F_NEXT_DONE : F_PREV_DONE;
+        const uint32_t wait = side ? F_NEXT_WAIT : F_PREV_WAIT;
+        if (flags & (done | wait)) continue;
+        if (s.found[side] != NONE) {
+            flags |= done;
+        } else if (s.cursor[side] >= s.len[side]) {
+            flags |= wait;
+            code |= 2u << side;
+        }
+    }
+    s.flags = flags;
+    if ((flags & (F_PREV_DONE | F_NEXT_DONE)) == (F_PREV_DONE | F_NEXT_DONE)) code |= 1u;
+    if (!code) return;
+    Event ev;
+    ev.slot = slot;
+    ev.code = code;
+    for (int side = 0; side < 2; ++side) {
+        ev.found[side] = s.found[side];
+        ev.x[side] = s.found[side] != NONE ? lists[((size_t)slot * 2 + side) * cap + s.found[side]] : 0;
+    }
+    ev.pad[0] = ev.pad[1] = 0;
+    events[atomicAdd(&ctr[2], 1u)] = ev;
+}
+
+// ---- host side --------------------------------------------------------------
+
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct SlotHost {
+    bool used = false;
+    int family = 0;
+    uint64_t m = 0, tag = 0;
+    uint32_t orig_len[2] = {0, 0};
+    uint32_t len[2] = {0, 0};
+    uint32_t fb_from[2] = {0, 0};  // next offset the fallback screen starts at
+    bool fb[2] = {false, false};
+    double t_submit = 0;
+    uint32_t bits = 0;             // bit length of the centre
+};
+
+struct Staged {
+    int family;
+    uint64_t m, tag;
+    std::vector<int32_t> prev, next;
+    double t_submit;
+};
+
+class Scanner {
+  public:
+    pgap_engine *e;
+    uint32_t depth = 8, confirm_rounds = 2;
+    uint32_t S = 0, CW = 0, CAP = 0, W = 0, MAXI = 0;
+    bool inited = false;
+
+    std::mutex mu;                 // guards `staged` (submit thread vs poll thread)
+    std::deque<Staged> staged;
+    std::vector<SlotHost> slots;
+    std::vector<uint32_t> free_slots, active;
+    std::vector<pgap_scan_row> ready;
+    std::vector<Upload> pending_append;      // fallback appends for the next round
+    std::vector<std::vector<uint32_t>> pending_append_data;
+
+    // family small-prime residues for the fallback screen
+    std::vector<uint32_t> fb_primes;
+    std::vector<std::vector<uint32_t>> fb_kmod;  // per family
+
+    SlotState *d_state = nullptr;
+    uint32_t *d_center = nullptr, *d_active = nullptr, *d_items = nullptr, *d_idx = nullptr, *d_nbuf = nullptr,
+             *d_hits = nullptr, *d_ctr = nullptr, *d_updata = nullptr;
+    int32_t *d_lists = nullptr;
+    uint8_t *d_verdict = nullptr;
+    Event *d_events = nullptr;
+    Upload *d_up = nullptr;
+    size_t up_cap = 0, updata_cap = 0;
+
+    // pinned host staging
+    uint32_t *h_active = nullptr, *h_ctr = nullptr;
+    Event *h_events = nullptr;
+    Upload *h_up = nullptr;
+    uint32_t *h_updata = nullptr;
+
+    uint64_t tests_issued = 0, rounds = 0;
+
+    explicit Scanner(pgap_engine *eng) : e(eng) {}
+    ~Scanner() { release(); }
+
+    void release() {
+        (void)hipStreamSynchronize(e->stream);
+        for (void *p : {(void *)d_state, (void *)d_center, (void *)d_active, (void *)d_items, (void *)d_idx,
+                        (void *)d_nbuf, (void *)d_hits, (void *)d_ctr, (void *)d_lists, (void *)d_verdict,
+                        (void *)d_events, (void *)d_up, (void *)d_updata})
+            if (p) (void)hipFree(p);
+        for (void *p : {(void *)h_active, (void *)h_ctr, (void *)h_events, (void *)h_up, (void *)h_updata})
+            if (p) (void)hipHostFree(p);
+        d_state = nullptr;
+        d_center = d_active = d_items = d_idx = d_nbuf = d_hits = d_ctr = d_updata = nullptr;
+        d_lists = nullptr;
+        d_verdict = nullptr;
+        d_events = nullptr;
+        d_up = nullptr;
+        h_active = h_ctr = h_updata = nullptr;
+        h_events = nullptr;
+        h_up = nullptr;
+        inited = false;
+    }
+
+    size_t in_flight() {
+        std::lock_guard<std::mutex> g(mu);
+        return staged.size() + (active.size()) + ready.size();
+    }
+
+    template <class T>
+    pgap_status dalloc(T **p, size_t n, const char *what) {
+        hipError_t he = hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+        if (he != hipSuccess) {
+            (void)hipGetLastError();
+            e->set_error("scan: hipMalloc %s (%zu bytes): %s", what, n * sizeof(T), hipGetErrorString(he));
+            return PGAP_E_OOM;
+        }
+        return PGAP_OK;
+    }
+    template <class T>
+    pgap_status halloc(T **p, size_t n, const char *what) {
+        hipError_t he = hipHostMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T), 0);
+        if (he != hipSuccess) {
+            (void)hipGetLastError();
+            e->set_error("scan: hipHostMalloc %s: %s", what, hipGetErrorString(he));
+            return PGAP_E_OOM;
+        }
+        return PGAP_OK;
+    }
+
+    // Size the slot arena from the families registered so far and the budget.
+    pgap_status init() {
+        if (inited) return PGAP_OK;
+        if (e->families.empty()) {
+            e->set_error("scan: register a family first (pgap_family)");
+            return PGAP_E_STATE;
+        }
+        uint32_t kw = 0, sl = 0;
+        for (const Family &f : e->families) {
+            kw = std::max<uint32_t>(kw, (uint32_t)f.K.size());
+            sl = std::max(sl, f.SL);
+        }
+        CW = kw + 2;   // m < 2^64
+        W = CW + 1;    // room for the carry of + x
+        CAP = std::max<uint32_t>(sl, 1) + 4 * FALLBACK_CHUNK;
+        const size_t per_slot = sizeof(SlotState) + 4ull * CW + 8ull * CAP + 8 /*active*/ + sizeof(Event) +
+                                2ull * depth_max() * (4ull * W + 4 + 4 + 1 + 4);
+        size_t budget = e->opt.hbm_budget;
+        if (budget == 0) {
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)8 << 30;
+            budget = fr / 4;
+        }
+        S = (uint32_t)std::min<size_t>(MAX_SLOTS, budget / per_slot);
+        if (S == 0) {
+            e->set_error("scan: HBM budget %zu bytes cannot hold one m (%zu bytes per slot)", budget, per_slot);
+            return PGAP_E_TOO_LARGE;
+        }
+        MAXI = 2 * S * depth_max();
+        pgap_status st;
+        if ((st = dalloc(&d_state, S, "state")) || (st = dalloc(&d_center, (size_t)S * CW, "centres")) ||
+            (st = dalloc(&d_lists, (size_t)S * 2 * CAP, "lists")) || (st = dalloc(&d_active, S, "active")) ||
+            (st = dalloc(&d_items, MAXI, "items")) || (st = dalloc(&d_idx, MAXI, "item index")) ||
+            (st = dalloc(&d_nbuf, (size_t)MAXI * W, "numbers")) || (st = dalloc(&d_verdict, MAXI, "verdicts")) ||
+            (st = dalloc(&d_hits, MAXI, "hits")) || (st = dalloc(&d_ctr, 4, "counters")) ||
+            (st = dalloc(&d_events, S, "events")) || (st = halloc(&h_active, S, "active")) ||
+            (st = halloc(&h_ctr, 4, "counters")) || (st = halloc(&h_events, S, "events"))) {
+            release();
+            return st;
+        }
+        slots.assign(S, SlotHost());
+        free_slots.clear();
+        for (uint32_t s = S; s-- > 0;) free_slots.push_back(s);
+        // small primes for the beyond-SL screen (K mod p per family on demand)
+        fb_primes = primes_upto(1u << 14);
+        fb_kmod.clear();
+        inited = true;
+        return PGAP_OK;
+    }
+
+    const std::vector<uint32_t> &kmod_small(int family) {
+        while (fb_kmod.size() <= (size_t)family) {
+            const Family &f = e->families[fb_kmod.size()];
+            std::vector<uint32_t> km(fb_primes.size());
+            for (size_t i = 0; i < fb_primes.size(); ++i) km[i] = mod_small(f.K, fb_primes[i]);
+            fb_kmod.push_back(std::move(km));
+        }
+        return fb_kmod[(size_t)family];
+    }
+
+    uint32_t depth_max() const { return std::max<uint32_t>(depth, 1); }
+
+    pgap_status ensure_upload(size_t nup, size_t nwords) {
+        if (nup > up_cap) {
+            if (d_up) (void)hipFree(d_up);
+            if (h_up) (void)hipHostFree(h_up);
+            d_up = nullptr;
+            h_up = nullptr;
+            up_cap = 0;
+            const size_t c = std::max<size_t>(nup, 256);
+            pgap_status st;
+            if ((st = dalloc(&d_up, c, "upload descriptors")) || (st = halloc(&h_up, c, "upload descriptors")))
+                return st;
+            up_cap = c;
+        }
+        if (nwords > updata_cap) {
+            if (d_updata) (void)hipFree(d_updata);
+            if (h_updata) (void)hipHostFree(h_updata);
+            d_updata = nullptr;
+            h_updata = nullptr;
+            updata_cap = 0;
+            const size_t c = std::max<size_t>(nwords, (size_t)1 << 20);
+            pgap_status st;
+            if ((st = dalloc(&d_updata, c, "upload data")) || (st = halloc(&h_updata, c, "upload data")))
+                return st;
+            updata_cap = c;
+        }
+        return PGAP_OK;
+    }
+
+    // Screen offsets beyond SL for one exhausted side: no factor below 2^14.
+    std::vector<uint32_t> fallback_candidates(SlotHost &sh, int side) {
+        const std::vector<uint32_t> &km = kmod_small(sh.family);
+        std::vector<uint32_t> out;
+        uint32_t x = sh.fb_from[side];
+        const uint32_t end = x + FALLBACK_SPAN;
+        std::vector<uint32_t> mr(fb_primes.size());
+        for (size_t i = 0; i < fb_primes.size(); ++i)
+            mr[i] = (uint32_t)((sh.m % fb_primes[i]) * km[i] % fb_primes[i]);
+        for (; x < end && out.size() < FALLBACK_CHUNK; ++x) {
+            bool ok = true;
+            for (size_t i = 0; i < fb_primes.size() && ok; ++i) {
+                const uint32_t p = fb_primes[i];
+                const uint32_t xr = x % p;
+                // N = m*K - x (prev) or m*K + x (next): divisible iff m*K == +-x mod p
+                ok = side ? ((mr[i] + xr) % p != 0) : (mr[i] != xr);
+            }
+            if (ok) out.push_back(x);
+        }
+        sh.fb_from[side] = x;
+        return out;
+    }
+
+    pgap_status admit() {
+        std::vector<Staged> take;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            while (!staged.empty() && take.size() < free_slots.size()) {
+                take.push_back(std::move(staged.front()));
+                staged.pop_front();
+            }
+        }
+        const size_t nup = take.size() + pending_append.size();
+        if (nup == 0) return PGAP_OK;
+        size_t words = 0;
+        for (const Staged &r : take) words += CW + r.prev.size() + r.next.size();
+        for (const auto &d : pending_append_data) words += d.size();
+        pgap_status st;
+        if ((st = ensure_upload(nup, words))) return st;
+        size_t u = 0, off = 0;
+        for (Staged &r : take) {
+            const uint32_t slot = free_slots.back();
+            free_slots.pop_back();
+            const Family &f = e->families[(size_t)r.family];
+            Words c = f.K;
+            mul_small(c, r.m);
+            SlotHost &sh = slots[slot];
+            sh = SlotHost();
+            sh.used = true;
+            sh.family = r.family;
+            sh.m = r.m;
+            sh.tag = r.tag;
+            sh.orig_len[0] = sh.len[0] = (uint32_t)r.prev.size();
+            sh.orig_len[1] = sh.len[1] = (uint32_t)r.next.size();
+            sh.fb_from[0] = sh.fb_from[1] = f.SL + 1;
+            sh.t_submit = r.t_submit;
+            sh.bits = bitlen(c);
+            Upload d{};
+            d.slot = slot;
+            d.kind = 0;
+            d.cwords = (uint32_t)c.size();
+            d.n0 = sh.len[0];
+            d.n1 = sh.len[1];
+            d.off_center = (uint32_t)off;
+            memcpy(h_updata + off, c.data(), c.size() * 4);
+            off += CW;
+            d.off_l0 = (uint32_t)off;
+            memcpy(h_updata + off, r.prev.data(), r.prev.size() * 4);
+            off += r.prev.size();
+            d.off_l1 = (uint32_t)off;
+            memcpy(h_updata + off, r.next.data(), r.next.size() * 4);
+            off += r.next.size();
+            h_up[u++] = d;
+            active.push_back(slot);
+        }
+        for (size_t i = 0; i < pending_append.size(); ++i) {
+            Upload d = pending_append[i];
+            d.off_l0 = (uint32_t)off;
+            const auto &v = pending_append_data[i];
+            memcpy(h_updata + off, v.data(), v.size() * 4);
+            off += v.size();
+            h_up[u++] = d;
+        }
+        pending_append.clear();
+        pending_append_data.clear();
+        hipError_t he;
+        if ((he = hipMemcpyAsync(d_up, h_up, nup * sizeof(Upload), hipMemcpyHostToDevice, e->stream)) !=
+                hipSuccess ||
+            (he = hipMemcpyAsync(d_updata, h_updata, std::max<size_t>(off, 1) * 4, hipMemcpyHostToDevice,
+                                 e->stream)) != hipSuccess)
+            return e->hip_fail(he, "scan upload");
+        hipLaunchKernelGGL(scatter_kernel, dim3((uint32_t)nup), dim3(256), 0, e->stream, d_up, d_updata,
+                           (uint32_t)nup, d_state, d_center, CW, d_lists, CAP);
+        if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "scatter kernel");
+        return PGAP_OK;
+    }
+
+    pgap_status round() {
+        pgap_status st;
+        if ((st = admit())) return st;
+        if (active.empty()) return PGAP_OK;
+        const uint32_t na = (uint32_t)active.size();
+        // speculative depth: fill about one wave per SIMD when few sides are
+        // active, one test per side... up to `depth` when the device is busy
+        const uint32_t target = 65536;
+        uint32_t d = std::max<uint32_t>(1, std::min<uint32_t>(depth_max(), (target + 2 * na - 1) / (2 * na)));
+        uint32_t bits = 0;
+        for (uint32_t s : active) bits = std::max(bits, slots[s].bits);
+        bits += 1;  // + x
+        const Variant *vf, *vc = nullptr;
+        if ((st = e->pick(KIND_FERMAT2, bits, &vf))) return st;
+        if (confirm_rounds && (st = e->pick(KIND_SPRP, bits, &vc))) return st;
+        Bases none{}, conf{};
+        static const uint32_t mr_bases[kMaxBases] = {3, 5, 7, 11, 13, 17, 19, 23};
+        conf.n = std::min<uint32_t>(confirm_rounds, kMaxBases);
+        for (uint32_t i = 0; i < conf.n; ++i) conf.b[i] = mr_bases[i];
+
+        memcpy(h_active, active.data(), na * 4);
+        hipError_t he;
+        if ((he = hipMemcpyAsync(d_active, h_active, na * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
+            (he = hipMemsetAsync(d_ctr, 0, 16, e->stream)) != hipSuccess)
+            return e->hip_fail(he, "scan round set-up");
+        const uint32_t maxi = 2 * na * d;
+        const dim3 b256(256);
+        hipLaunchKernelGGL(issue_kernel, dim3((2 * na + 255) / 256), b256, 0, e->stream, d_active, na, d_state, d,
+                           d_items, d_idx, d_ctr);
+        hipLaunchKernelGGL(materialize_kernel, dim3((maxi + 255) / 256), b256, 0, e->stream, d_items, d_idx, d_ctr,
+                           d_center, CW, d_lists, CAP, d_nbuf, W);
+        if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "scan issue");
+        if ((he = vf->launch(d_nbuf, W, maxi, d_ctr, nullptr, none, d_verdict, e->opt.waves_per_simd, e->stream)) !=
+            hipSuccess)
+            return e->hip_fail(he, "scan Fermat launch");
+        if (vc) {
+            hipLaunchKernelGGL(hits_kernel, dim3((maxi + 255) / 256), b256, 0, e->stream, d_verdict, d_ctr, d_hits);
+            if ((he = vc->launch(d_nbuf, W, maxi, d_ctr + 1, d_hits, conf, d_verdict, e->opt.waves_per_simd,
+                                 e->stream)) != hipSuccess)
+                return e->hip_fail(he, "scan confirm launch");
+        }
+        hipLaunchKernelGGL(retire_kernel, dim3((maxi + 255) / 256), b256, 0, e->stream, d_items, d_idx, d_verdict,
+                           d_ctr, d_state);
+        hipLaunchKernelGGL(update_kernel, dim3((na + 255) / 256), b256, 0, e->stream, d_active, na, d_state, d_lists,
+                           CAP, d_ctr, d_events);
+        if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "scan retire");
+        if ((he = hipMemcpyAsync(h_ctr, d_ctr, 16, hipMemcpyDeviceToHost, e->stream)) != hipSuccess ||
+            (he = hipStreamSynchronize(e->stream)) != hipSuccess)
+            return e->hip_fail(he, "scan round");
+        const uint32_t nev = h_ctr[2];
+        tests_issued += h_ctr[0];
+        ++rounds;
+        if (nev) {
+            if ((he = hipMemcpy(h_events, d_events, nev * sizeof(Event), hipMemcpyDeviceToHost)) != hipSuccess)
+                return e->hip_fail(he, "scan events");
+        }
+        const double t = now_s();
+        std::vector<uint32_t> finished;
+        for (uint32_t i = 0; i < nev; ++i) {
+            const Event &ev = h_events[i];
+            SlotHost &sh = slots[ev.slot];
+            for (int side = 0; side < 2; ++side) {
+                if (!(ev.code & (2u << side))) continue;
+                // exhausted: append screened candidates beyond SL (or past the last chunk)
+                sh.fb[side] = true;
+                std::vector<uint32_t> c = fallback_candidates(sh, side);
+                while (c.empty()) c = fallback_candidates(sh, side);
+                Upload d{};
+                d.slot = ev.slot;
+                d.kind = 1;
+                d.side = (uint32_t)side;
+                d.n0 = (uint32_t)c.size();
+                if (sh.len[side] + d.n0 > CAP) {
+                    e->set_error("scan: m=%llu side %d ran past %u fallback candidates",
+                                 (unsigned long long)sh.m, side, CAP);
+                    return PGAP_E_STATE;
+                }
+                sh.len[side] += d.n0;
+                pending_append.push_back(d);
+                pending_append_data.push_back(std::move(c));
+            }
+            if (ev.code & 1u) {
+                pgap_scan_row r{};
+                r.m = sh.m;
+                r.tag = sh.tag;
+                r.family = (uint32_t)sh.family;
+                r.prev_p = ev.x[0];
+                r.next_p = ev.x[1];
+                r.p_tests = std::min(ev.found[0] + 1, sh.orig_len[0]);
+                r.n_tests = std::min(ev.found[1] + 1, sh.orig_len[1]);
+                r.flags = (sh.fb[0] ? PGAP_ROW_PREV_FALLBACK : 0) | (sh.fb[1] ? PGAP_ROW_NEXT_FALLBACK : 0);
+                r.t_submit = sh.t_submit;
+                r.t_done = t;
+                ready.push_back(r);
+                finished.push_back(ev.slot);
+            }
+        }
+        if (!finished.empty()) {
+            std::sort(finished.begin(), finished.end());
+            std::vector<uint32_t> keep;
+            keep.reserve(active.size());
+            for (uint32_t s : active)
+                if (!std::binary_search(finished.begin(), finished.end(), s)) keep.push_back(s);
+            active.swap(keep);
+            for (uint32_t s : finished) {
+                slots[s].used = false;
+                free_slots.push_back(s);
+            }
+        }
+        return PGAP_OK;
+    }
+};
+
 }  // namespace pgap
+
+using namespace pgap;
 
 void pgap_engine::scan_shutdown() {
     delete scanner;
     scanner = nullptr;
 }
 
+static Scanner *get_scanner(pgap_engine *e) {
+    if (!e->scanner) e->scanner = new Scanner(e);
+    return e->scanner;
+}
+
 extern "C" {
+
+pgap_status pgap_scan_config(pgap_engine *e, uint32_t depth, uint32_t confirm_rounds) {
+    if (!e) return PGAP_E_ARG;
+    if (!e->ready) return PGAP_E_STATE;
+    Scanner *s = get_scanner(e);
+    if (s->inited) {
+        e->set_error("pgap_scan_config: must be called before the first submit");
+        return PGAP_E_STATE;
+    }
+    if (depth == 0 || depth > 256 || confirm_rounds > (uint32_t)kMaxBases) {
+        e->set_error("pgap_scan_config: depth in [1,256], confirm_rounds in [0,%d]", kMaxBases);
+        return PGAP_E_ARG;
+    }
+    s->depth = depth;
+    s->confirm_rounds = confirm_rounds;
+    return PGAP_OK;
+}
+
+pgap_status pgap_scan_submit(pgap_engine *e, const pgap_scan_req *req) {
+    if (!e || !req) return PGAP_E_ARG;
+    if (!e->ready) return PGAP_E_STATE;
+    Family *f = e->family(req->family);
+    if (!f || req->m == 0 || (req->n_prev && !req->prev) || (req->n_next && !req->next)) {
+        e->set_error("pgap_scan_submit: bad family, m or list");
+        return PGAP_E_ARG;
+    }
+    Scanner *s = get_scanner(e);
+    (void)hipSetDevice(e->opt.device);
+    pgap_status st = s->init();
+    if (st) return st;
+    if ((size_t)f->K.size() + 2 > s->CW) {
+        e->set_error("pgap_scan_submit: family registered after the scan started is wider than its slots");
+        return PGAP_E_STATE;
+    }
+    if (req->n_prev > f->SL || req->n_next > f->SL) {
+        e->set_error("pgap_scan_submit: more unknowns than the sieve length allows");
+        return PGAP_E_TOO_LARGE;
+    }
+    for (uint32_t i = 0; i < req->n_prev; ++i)
+        if (req->prev[i] <= 0 || (uint32_t)req->prev[i] > f->SL) {
+            e->set_error("pgap_scan_submit: prev distance %d outside [1, SL]", req->prev[i]);
+            return PGAP_E_ARG;
+        }
+    for (uint32_t i = 0; i < req->n_next; ++i)
+        if (req->next[i] <= 0 || (uint32_t)req->next[i] > f->SL) {
+            e->set_error("pgap_scan_submit: next distance %d outside [1, SL]", req->next[i]);
+            return PGAP_E_ARG;
+        }
+    Staged r;
+    r.family = req->family;
+    r.m = req->m;
+    r.tag = req->tag;
+    r.prev.assign(req->prev, req->prev + req->n_prev);
+    r.next.assign(req->next, req->next + req->n_next);
+    r.t_submit = now_s();
+    std::lock_guard<std::mutex> g(s->mu);
+    // admission control: staged requests wait for a free slot; refuse when the
+    // backlog already fills every slot once more
+    if (s->staged.size() >= s->S) {
+        e->set_error("pgap_scan_submit: HBM budget full (%u slots busy, %zu waiting)", s->S, s->staged.size());
+        return PGAP_E_OOM;
+    }
+    s->staged.push_back(std::move(r));
+    return PGAP_OK;
+}
+
+pgap_status pgap_scan_poll(pgap_engine *e, pgap_scan_row *rows, size_t cap, size_t *n_out, int wait) {
+    if (!e || !n_out || (cap && !rows)) return PGAP_E_ARG;
+    if (!e->ready) return PGAP_E_STATE;
+    *n_out = 0;
+    Scanner *s = get_scanner(e);
+    if (!s->inited) return PGAP_OK;
+    (void)hipSetDevice(e->opt.device);
+    for (;;) {
+        if (!s->ready.empty() || (s->active.empty() && s->in_flight() == 0)) break;
+        pgap_status st = s->round();
+        if (st) return st;
+        if (!wait) break;
+        if (s->active.empty() && s->pending_append.empty()) {
+            std::lock_guard<std::mutex> g(s->mu);
+            if (s->staged.empty()) break;
+        }
+    }
+    const size_t n = std::min(cap, s->ready.size());
+    for (size_t i = 0; i < n; ++i) rows[i] = s->ready[i];
+    s->ready.erase(s->ready.begin(), s->ready.begin() + (long)n);
+    *n_out = n;
+    return PGAP_OK;
+}
+
+size_t pgap_scan_in_flight(const pgap_engine *e) {
+    if (!e || !e->scanner) return 0;
+    return e->scanner->in_flight();
+}
+
 pgap_status pgap_prp_batch_mixed(pgap_engine *e, const uint32_t *, const uint64_t *, const uint32_t *, size_t,
                                  pgap_test, const uint32_t *, uint32_t, uint8_t *) {
     if (!e) return PGAP_E_ARG;
     e->set_error("mixed batches: not built yet");
     return PGAP_E_STATE;
 }
-pgap_status pgap_scan_submit(pgap_engine *e, const pgap_scan_req *) {
-    if (!e) return PGAP_E_ARG;
-    e->set_error("scan: not built yet");
-    return PGAP_E_STATE;
-}
-pgap_status pgap_scan_poll(pgap_engine *e, pgap_scan_row *, size_t, size_t *n_out, int) {
-    if (!e) return PGAP_E_ARG;
-    if (n_out) *n_out = 0;
-    e->set_error("scan: not built yet");
-    return PGAP_E_STATE;
-}
-size_t pgap_scan_in_flight(const pgap_engine *) { return 0; }
-pgap_status pgap_scan_config(pgap_engine *e, uint32_t, uint32_t) {
-    if (!e) return PGAP_E_ARG;
-    return PGAP_E_STATE;
-}
-}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+"""Streaming scan parity: per-m rows from the device-resident scan against
+the oracle's restatement of test_interval_cpu (gap_test_common.cpp:128-181),
+which is itself pinned to the reference README rows."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import pgap
+import unknown_format as uf
+
+pytestmark = pytest.mark.gpu
+
+
+def lines_907(unknowns_907):
+    return [uf.parse_line(ln) for ln in unknowns_907.splitlines(keepends=True)]
+
+
+def run_scan(engine, fam, reqs, depth=None, confirm=None):
+    """Submit everything (retrying on admission back-pressure), collect rows."""
+    rows = {}
+    pending = list(reqs)
+    while pending or engine.scan_in_flight():
+        while pending:
+            m, p, n = pending[0]
+            st = engine.scan_submit(fam, m, np.array(p, np.int32), np.array(n, np.int32), tag=m)
+            if st == pgap.PGAP_E_OOM:
+                break
+            pending.pop(0)
+        for r in engine.scan_poll(wait=True):
+            assert r.m not in rows
+            rows[r.m] = r
+    return rows
+
+
+def test_scan_rows_907_match_oracle(unknowns_907):
+    lines = lines_907(unknowns_907)
+    with pgap.Engine(0) as e:
+        fam = e.family(907, 2190, 11000)
+        rows = run_scan(e, fam, lines)
+    assert len(rows) == len(lines)
+    total = 0
+    for m, p, n in lines:
+        want = ol.test_interval(907, 2190, m, 11000, p, n, kind=0)
+        r = rows[m]
+        assert (r.prev_p, r.next_p) == (want["prev_p"], want["next_p"]), m
+        assert (r.p_tests, r.n_tests) == (want["p_tests"], want["n_tests"]), m
+        assert r.flags == 0 and r.tag == m
+        total += r.p_tests + r.n_tests
+    assert total == 2590  # README.md:241-262 prp-test total over the 53 m's
+
+
+@pytest.mark.parametrize("depth,confirm", [(1, 0), (3, 1), (16, 2)])
+def test_scan_depth_and_confirmation_do_not_change_rows(unknowns_907, depth, confirm):
+    lines = lines_907(unknowns_907)[:20]
+    with pgap.Engine(0) as e:
+        e.scan_config(depth, confirm)
+        fam = e.family(907, 2190, 11000)
+        rows = run_scan(e, fam, lines)
+    for m, p, n in lines:
+        want = ol.test_interval(907, 2190, m, 11000, p, n, kind=0)
+        assert (rows[m].prev_p, rows[m].next_p, rows[m].p_tests, rows[m].n_tests) == \
+            (want["prev_p"], want["next_p"], want["p_tests"], want["n_tests"])
+
+
+def test_alone_equals_mixed(unknowns_907):
+    lines = lines_907(unknowns_907)
+    with pgap.Engine(0) as e:
+        fam = e.family(907, 2190, 11000)
+        mixed = run_scan(e, fam, lines)
+        alone = {}
+        for m, p, n in lines[::7]:
+            alone.update(run_scan(e, fam, [(m, p, n)]))
+    for m, r in alone.items():
+        a = (r.prev_p, r.next_p, r.p_tests, r.n_tests, r.flags)
+        b = (mixed[m].prev_p, mixed[m].next_p, mixed[m].p_tests, mixed[m].n_tests, mixed[m].flags)
+        assert a == b, m
+
+
+def test_fallback_beyond_sieve_length():
+    # README.md:40-46 record: 11244911 * 809#/1841070, next prime +17354 is
+    # beyond SL = 15000, so the next side runs out of unknowns and continues
+    # past SL (gap_test_common.cpp:170-178).
+    m, P, D, SL = 11244911, 809, 1841070, 15000
+    sv = ol.Sieve(P, D, SL, 1_000_000)
+    prev, nxt = sv(m)
+    want = ol.test_interval(P, D, m, SL, prev, nxt, kind=0)
+    assert (want["prev_p"], want["next_p"], want["next_fallback"]) == (5788, 17354, 1)
+    with pgap.Engine(0) as e:
+        fam = e.family(P, D, SL)
+        rows = run_scan(e, fam, [(m, prev, nxt)])
+        r = rows[m]
+        assert (r.prev_p, r.next_p) == (5788, 17354)
+        assert r.flags == pgap.ROW_NEXT_FALLBACK
+        assert (r.p_tests, r.n_tests) == (want["p_tests"], want["n_tests"])
+        # empty lists on both sides: both fall back
+        fam2 = e.family(P, D, 5000)
+        rows = run_scan(e, fam2, [(m, [], [])])
+        r = rows[m]
+        assert (r.prev_p, r.next_p, r.p_tests, r.n_tests) == (5788, 17354, 0, 0)
+        assert r.flags == pgap.ROW_PREV_FALLBACK | pgap.ROW_NEXT_FALLBACK
+
+
+def test_scan_2100_bit_family_small():
+    # 1511#/2190 (config-4 size) on a few m's sieved to 1e6
+    P, D, SL = 1511, 2190, 40000
+    sv = ol.Sieve(P, D, SL, 1_000_000)
+    ms = [m for m in range(1, 60) if math.gcd(m, D) == 1][:6]
+    reqs = [(m,) + sv(m) for m in ms]
+    with pgap.Engine(0) as e:
+        fam = e.family(P, D, SL)
+        rows = run_scan(e, fam, reqs)
+    for m, p, n in reqs:
+        want = ol.test_interval(P, D, m, SL, p, n, kind=1)
+        r = rows[m]
+        assert (r.prev_p, r.next_p, r.p_tests, r.n_tests) == \
+            (want["prev_p"], want["next_p"], want["p_tests"], want["n_tests"]), m
+
+
+def test_budget_and_admission(unknowns_907):
+    lines = lines_907(unknowns_907)
+    with pgap.Engine(0, hbm_budget=1000) as e:
+        fam = e.family(907, 2190, 11000)
+        with pytest.raises(pgap.PgapError) as ei:
+            e.scan_submit(fam, 1, np.array([2], np.int32), np.array([4], np.int32))
+        assert ei.value.status == pgap.PGAP_E_TOO_LARGE
+    # room for a handful of m's: submissions beyond the backlog are refused
+    # with PGAP_E_OOM and succeed after polling
+    with pgap.Engine(0, hbm_budget=3 << 20) as e:
+        fam = e.family(907, 2190, 11000)
+        refused = 0
+        rows = {}
+        pending = list(lines)
+        while pending or e.scan_in_flight():
+            while pending:
+                m, p, n = pending[0]
+                if e.scan_submit(fam, m, np.array(p, np.int32), np.array(n, np.int32)) == pgap.PGAP_E_OOM:
+                    refused += 1
+                    break
+                pending.pop(0)
+            for r in e.scan_poll(wait=True):
+                rows[r.m] = r
+        assert refused > 0 and len(rows) == len(lines)
+        for m, p, n in lines[:10]:
+            want = ol.test_interval(907, 2190, m, 11000, p, n, kind=0)
+            assert (rows[m].prev_p, rows[m].next_p) == (want["prev_p"], want["next_p"])
+
+
+def test_submit_validates_lists():
+    with pgap.Engine(0) as e:
+        fam = e.family(907, 2190, 11000)
+        with pytest.raises(pgap.PgapError):
+            e.scan_submit(fam, 1, np.array([0], np.int32), np.array([], np.int32))
+        with pytest.raises(pgap.PgapError):
+            e.scan_submit(fam, 1, np.array([11001], np.int32), np.array([], np.int32))
+        with pytest.raises(pgap.PgapError):
+            e.scan_submit(fam + 5, 1, np.array([2], np.int32), np.array([], np.int32))
