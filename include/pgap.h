@@ -124,6 +124,16 @@ pgap_status pgap_build_candidates_device(pgap_engine *e, int family, const uint6
 pgap_status pgap_filter_candidates(pgap_engine *e, int family, const uint64_t *m, const int64_t *x,
                                    size_t count, uint32_t limit, uint8_t *keep);
 
+/* Device sieve: the unknowns combined_sieve writes (combined_sieve.cpp:
+ * 1261-1445), i.e. the offsets x in [-SL, SL], x != 0, for which m*K + x has
+ * no prime factor <= max_prime (SL = the family's sieve length).  For each
+ * m_i the prev distances go to prev + i*SL (increasing), the next distances
+ * to next + i*SL, and their counts to n_prev[i], n_next[i] (host arrays).
+ * The primes <= max_prime are generated on the device and kept for later
+ * calls with the same limit.  max_prime <= 4e9. */
+pgap_status pgap_sieve(pgap_engine *e, int family, const uint64_t *m, size_t count, uint64_t max_prime,
+                       int32_t *prev, uint32_t *n_prev, int32_t *next, uint32_t *n_next);
+
 typedef struct {
     int family;
     uint64_t m;

@@ -234,6 +234,7 @@ void pgap_destroy(pgap_engine *e) {
         (void)hipSetDevice(e->opt.device);
         (void)hipStreamSynchronize(e->stream);
         e->scan_shutdown();
+        e->sieve_shutdown();
         e->families_shutdown();
         for (DevBuf *b : {&e->d_n, &e->d_out, &e->d_small, &e->d_m, &e->d_x, &e->d_keep, &e->d_idx0, &e->d_idx1}) {
             if (b->p) (void)hipFree(b->p);

@@ -33,6 +33,9 @@ struct Family {
     uint32_t *d_kmod = nullptr;  // K mod p
     uint64_t *d_mu = nullptr;    // floor((2^64-1)/p) for Barrett reduction
     uint32_t stage_end[2] = {0, 0};  // prime-index ends of the first two filter stages
+    // K mod p over the engine's sieve prime table (pgap_sieve)
+    uint32_t *d_skmod = nullptr;
+    uint64_t skmod_limit = 0;
 };
 }  // namespace pgap
 
@@ -50,6 +53,13 @@ struct pgap_engine {
     pgap::DevBuf d_n, d_out, d_small, d_m, d_x, d_keep, d_idx0, d_idx1;
     std::vector<pgap::Family> families;
     pgap::Scanner *scanner = nullptr;  // owned; released by scan_shutdown()
+
+    // primes <= sieve_limit (unordered) with Barrett constants, for pgap_sieve
+    uint32_t *sieve_primes = nullptr;
+    uint64_t *sieve_mu = nullptr;
+    size_t sieve_nprimes = 0;
+    uint64_t sieve_limit = 0;
+    void sieve_shutdown();
 
     pgap::Family *family(int id) {
         return (id >= 0 && (size_t)id < families.size()) ? &families[(size_t)id] : nullptr;

@@ -1,0 +1,372 @@
+// gap_test_hip -- drop-in for the reference testers gap_test_simple /
+// gap_test_gpu: reads a combined_sieve unknown file, finds the previous and
+// next prime around every m*P#/d on the GPU (streaming scan of libpgap) and
+// prints the same per-m rows and stats lines.
+//
+// Reference counterparts: main + prime_gap_test of gap_test_simple.cpp:43-200
+// (rows "%-5d %.4f  %ld * %ld#/%ld -%d to +%d", merit = gap / (K_log +
+// log m) as float), StatsCounters (gap_test_common.cpp:29-124), argument
+// handling of Args::argparse for the testers (gap_common.cpp:805-1099).
+//
+// Differences by design: PRP tests run on the GPU (Fermat base 2 plus
+// strong-PRP confirmation of each side's first hit, bases 3 and 5 by
+// default) instead of mpz_probab_prime_p(N, 25); m's are scanned concurrently
+// and rows are printed in m order as they complete; malformed input is an
+// error message and exit status 1, not an assert.
+//
+// Extra flags: --depth N, --confirm N (scan knobs), --hbm-budget-mb N,
+// --results FILE (tab-separated P D m next_p prev_p merit, the columns of the
+// reference's result table, schema.sql:142-154), --dump-unknowns (parse and
+// print the lists, no GPU).
+#include <getopt.h>
+
+#include <chrono>
+#include <clocale>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/pgap.h"
+#include "hostbn.h"
+#include "unknowns.h"
+
+using namespace pgap;
+
+namespace {
+
+struct Args {
+    UnknownConfig u;
+    float min_merit = 18;
+    uint64_t mskip = 0;
+    int verbose = 2;
+    uint32_t depth = 8, confirm = 2;
+    size_t hbm_budget = 0;
+    std::string results;
+    bool dump = false;
+    bool valid = true;
+};
+
+void usage(const char *name) {
+    printf("Usage: %s\n", name);
+    printf("[REQUIRED]\n  -u, --unknown-filename <P>_<D>_<mstart>_<minc>_s<SL>_l<MP>M.txt\n");
+    printf("    (or -p <p> -d <d> --mstart <m> --minc <n> --sieve-length <SL> --max-prime <MP>)\n\n");
+    printf("[OPTIONALLY]\n  --min-merit <x>   only display gaps with merit > x (default 18)\n");
+    printf("  --mskip <m>       skip m < mskip (partial resume)\n");
+    printf("  --depth <n>       speculative tests per side per round (default 8)\n");
+    printf("  --confirm <n>     strong-PRP bases confirming each side's first PRP (default 2)\n");
+    printf("  --hbm-budget-mb <n>  device memory the scan may keep resident\n");
+    printf("  --results <file>  append result rows (P D m next_p prev_p merit, tab separated)\n");
+    printf("  --dump-unknowns   print the parsed unknown lists and exit (no GPU)\n");
+    printf("  -q, --quiet       less output (twice for less)\n  -h, --help\n");
+}
+
+bool is_prime_small(uint64_t n) {
+    if (n < 2) return false;
+    for (uint64_t t = 2; t * t <= n; ++t)
+        if (n % t == 0) return false;
+    return true;
+}
+
+Args parse_args(int argc, char **argv) {
+    static struct option opts[] = {
+        {"unknown-filename", required_argument, 0, 'u'}, {"min-merit", required_argument, 0, 3},
+        {"mskip", required_argument, 0, 16},             {"p", required_argument, 0, 'p'},
+        {"d", required_argument, 0, 'd'},                {"mstart", required_argument, 0, 1},
+        {"minc", required_argument, 0, 2},               {"sieve-length", required_argument, 0, 4},
+        {"max-prime", required_argument, 0, 5},          {"depth", required_argument, 0, 20},
+        {"confirm", required_argument, 0, 21},           {"hbm-budget-mb", required_argument, 0, 22},
+        {"results", required_argument, 0, 23},           {"dump-unknowns", no_argument, 0, 24},
+        {"method1", no_argument, 0, 8},                  {"quiet", no_argument, 0, 'q'},
+        {"help", no_argument, 0, 'h'},                   {0, 0, 0, 0}};
+    Args a;
+    std::string err;
+    int c, idx = 0;
+    while ((c = getopt_long(argc, argv, "qhp:d:u:", opts, &idx)) >= 0) {
+        switch (c) {
+            case 'h': usage(argv[0]); exit(0);
+            case 'q': a.verbose--; break;
+            case 'p': a.u.p = (uint32_t)atoi(optarg); break;
+            case 'd': a.u.d = (uint32_t)atoi(optarg); break;
+            case 1: a.u.mstart = strtoull(optarg, nullptr, 10); break;
+            case 2: a.u.minc = strtoull(optarg, nullptr, 10); break;
+            case 4: a.u.sieve_length = (uint32_t)atoi(optarg); break;
+            case 5: a.u.max_prime = strtoull(optarg, nullptr, 10) * 1000000ull; break;
+            case 8: a.u.method1 = true; break;
+            case 16: a.mskip = strtoull(optarg, nullptr, 10); break;
+            case 3: a.min_merit = (float)atof(optarg); break;
+            case 20: a.depth = (uint32_t)atoi(optarg); break;
+            case 21: a.confirm = (uint32_t)atoi(optarg); break;
+            case 22: a.hbm_budget = (size_t)strtoull(optarg, nullptr, 10) << 20; break;
+            case 23: a.results = optarg; break;
+            case 24: a.dump = true; break;
+            case 'u':
+                if (!parse_unknown_filename(optarg, a.u, err)) {
+                    fprintf(stderr, "%s\n", err.c_str());
+                    a.valid = false;
+                }
+                break;
+            default: a.valid = false;
+        }
+    }
+    if (optind < argc) {
+        printf("unknown positional argument: %s\n", argv[optind]);
+        a.valid = false;
+    }
+    if (a.u.path.empty() && a.u.p) a.u.path = gen_unknown_fn(a.u);
+    if (a.u.mstart == 0) { printf("mstart must be greater than 0: %lu\n", (unsigned long)a.u.mstart); a.valid = false; }
+    if (a.u.minc == 0) { printf("minc must be greater than 0: %lu\n", (unsigned long)a.u.minc); a.valid = false; }
+    if (a.u.d == 0) { printf("d must be greater than 0: %u\n", a.u.d); a.valid = false; }
+    if (!is_prime_small(a.u.p)) { printf("p# not prime (p=%u)\n", a.u.p); a.valid = false; }
+    if (a.u.sieve_length == 0) { printf("Must set sieve-length\n"); a.valid = false; }
+    if (a.mskip && (a.mskip < a.u.mstart || a.mskip >= a.u.mstart + a.u.minc)) {
+        printf("mskip must lie in [mstart, mstart + minc)\n");
+        a.valid = false;
+    }
+    return a;
+}
+
+// StatsCounters (gap_test_common.cpp:29-124), restated.
+struct Stats {
+    std::chrono::steady_clock::time_point start = std::chrono::steady_clock::now();
+    uint32_t tests = 0;
+    size_t total_unknown = 0, unk_prev = 0, unk_next = 0, prp_tests = 0;
+    size_t fallback_prev = 0, fallback_next = 0;
+    float best_merit = 0;
+    long best_m = 0;
+    double tests_per_second = 0;
+
+    void process(int verbose, double SL, long m, bool last, size_t ul, size_t uu, int prev_p, int next_p,
+                 size_t pt, size_t nt, float merit) {
+        tests += 1;
+        total_unknown += ul + uu;
+        unk_prev += ul;
+        unk_next += uu;
+        prp_tests += pt + nt;
+        if (merit > best_merit) {
+            best_merit = merit;
+            best_m = m;
+        }
+        size_t interval = (size_t)(1800 * tests_per_second);
+        if (interval > 10000) interval -= interval % 10000;
+        bool power = tests == 1;
+        for (long p = 10; p <= (long)tests; p *= 10) power |= tests == p || tests == 3 * p || tests == 5 * p;
+        if (!(last || power || (interval > 0 && tests % interval == 0))) return;
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        tests_per_second = tests / secs;
+        if (verbose + (int)last < 1) return;
+        printf("\tm=%ld %4ld <- unknowns -> %-4ld\t%4d <- gap -> %-4d\n", m, (long)ul, (long)uu, prev_p, next_p);
+        if (tests > secs)
+            printf("\t    tests     %-10d (%.2f/sec)  %.0f seconds elapsed\n", tests, tests / secs, secs);
+        else
+            printf("\t    tests     %-10d (%.2f secs/test)  %.0f seconds elapsed\n", tests, secs / tests, secs);
+        printf("\t    unknowns  %-10ld (avg: %.2f), %.2f%% composite  %.2f%% <- %% -> %.2f%%\n", (long)total_unknown,
+               total_unknown / (double)tests, 100.0 * (1 - total_unknown / ((2.0 * SL + 1) * tests)),
+               100.0 * unk_prev / total_unknown, 100.0 * unk_next / total_unknown);
+        printf("\t    prp tests %-10ld (avg: %.2f) (%.1f tests/sec)\n", (long)prp_tests, prp_tests / (float)tests,
+               prp_tests / secs);
+        if (verbose >= 2) {
+            if (fallback_prev + fallback_next > 0)
+                printf("\t    fallback prev_gap %ld (%.2f%%), next_gap %ld (%.2f%%)\n", (long)fallback_prev,
+                       100.0 * fallback_prev / tests, (long)fallback_next, 100.0 * fallback_next / tests);
+            printf("\t    best merit this interval: %.3f (at m=%ld)\n", best_merit, best_m);
+        }
+        best_merit = 0;
+        best_m = 0;
+    }
+};
+
+struct Pending {
+    size_t ul = 0, uu = 0;
+};
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    Args a = parse_args(argc, argv);
+    if (!a.valid) {
+        usage(argv[0]);
+        return 1;
+    }
+    Words K;
+    if (!primorial_over(a.u.p, a.u.d, K)) {
+        fprintf(stderr, "d=%u does not divide %u#\n", a.u.d, a.u.p);
+        return 1;
+    }
+    const double K_log = ln_words(K);
+
+    setlocale(LC_NUMERIC, "");
+    if (a.verbose >= 2) printf("Compiled with %s\n", pgap_version());
+    if (a.verbose >= 0) {
+        printf("\n");
+        printf("Testing m * %u#/%u, m = %lu + [0, %'lu)\n", a.u.p, a.u.d, (unsigned long)a.u.mstart,
+               (unsigned long)a.u.minc);
+    }
+    if (a.mskip > 0) printf("\tskipping m < %'lu\n", (unsigned long)a.mskip);
+    setlocale(LC_NUMERIC, "C");
+
+    std::ifstream in(a.u.path, std::ios::in | std::ios::binary);
+    if (!in.is_open()) {
+        // a bare name resolves under unknowns/ first; accept it where it lies too
+        const std::string alt = a.u.path.substr(a.u.path.find('/') + 1);
+        in.open(alt, std::ios::in | std::ios::binary);
+        if (in.is_open()) a.u.path = alt;
+    }
+    if (!in.is_open()) {
+        fprintf(stderr, "can't open unknown file '%s'\n", a.u.path.c_str());
+        return 1;
+    }
+    if (a.verbose >= 1) printf("\nReading unknowns from '%s'\n", a.u.path.c_str());
+
+    std::string first;
+    if (!std::getline(in, first)) {
+        fprintf(stderr, "empty unknown file\n");
+        return 1;
+    }
+    const int format = detect_format(first);
+    uint64_t first_m = a.u.mstart;
+    while (gcd64(first_m, a.u.d) > 1) ++first_m;
+    if (format < 0 || strtoull(first.c_str(), nullptr, 10) != first_m) {
+        fprintf(stderr, "unknown file must start with the line of m=%lu (lines start with <mstart + mi>)\n",
+                (unsigned long)first_m);
+        return 1;
+    }
+    in.clear();
+    in.seekg(0);
+
+    BitLayout bl;
+    if (format == 2) bl.build(K, a.u.p, a.u.d, a.u.sieve_length);
+
+    uint64_t valid_ms = 0, last_mi = 0;
+    for (uint64_t mi = 0; mi < a.u.minc; ++mi)
+        if (gcd64(a.u.mstart + mi, a.u.d) == 1) {
+            ++valid_ms;
+            last_mi = mi;
+        }
+    const float min_merit = a.min_merit;
+    if (a.verbose >= 1) {
+        const float m_log = (float)log((double)a.u.mstart);
+        printf("Min Gap ~= %d (for merit > %.1f)\n", (int)(min_merit * (K_log + m_log)), min_merit);
+        printf("\n%lu tests M_start(%lu) + mi(%d to %lu)\n\n", (unsigned long)valid_ms, (unsigned long)a.u.mstart, 0,
+               (unsigned long)last_mi);
+    }
+
+    pgap_engine *e = nullptr;
+    int fam = -1;
+    if (!a.dump) {
+        pgap_options opt{};
+        opt.hbm_budget = a.hbm_budget;
+        if (pgap_create(&opt, &e) != PGAP_OK) {
+            fprintf(stderr, "pgap_create: %s\n", pgap_last_error(e));
+            pgap_destroy(e);
+            return 1;
+        }
+        if (pgap_scan_config(e, a.depth, a.confirm) != PGAP_OK ||
+            pgap_family(e, a.u.p, a.u.d, a.u.sieve_length, &fam) != PGAP_OK) {
+            fprintf(stderr, "scan set-up: %s\n", pgap_last_error(e));
+            pgap_destroy(e);
+            return 1;
+        }
+    }
+    FILE *results = nullptr;
+    if (!a.results.empty() && !(results = fopen(a.results.c_str(), "a"))) {
+        fprintf(stderr, "can't open %s\n", a.results.c_str());
+        return 1;
+    }
+
+    Stats stats;
+    std::map<uint64_t, Pending> pending;  // submitted, in m order
+    std::map<uint64_t, pgap_scan_row> done;
+    std::vector<pgap_scan_row> rows(4096);
+    int status = 0;
+
+    // print finished rows in m order
+    auto flush = [&](bool all) {
+        while (!pending.empty()) {
+            auto it = pending.begin();
+            auto d = done.find(it->first);
+            if (d == done.end()) break;
+            const pgap_scan_row &r = d->second;
+            const long m = (long)r.m;
+            const int prev_p = r.prev_p, next_p = r.next_p, gap = prev_p + next_p;
+            const float merit = (float)(gap / (K_log + log((double)m)));
+            if (merit > min_merit)
+                printf("%-5d %.4f  %ld * %u#/%u -%d to +%d\n", gap, merit, m, a.u.p, a.u.d, prev_p, next_p);
+            if (results)
+                fprintf(results, "%u\t%u\t%ld\t%d\t%d\t%.4f\n", a.u.p, a.u.d, m, next_p, prev_p, merit);
+            stats.fallback_prev += (r.flags & PGAP_ROW_PREV_FALLBACK) ? 1 : 0;
+            stats.fallback_next += (r.flags & PGAP_ROW_NEXT_FALLBACK) ? 1 : 0;
+            const bool is_last = (r.m - a.u.mstart) == last_mi;
+            stats.process(a.verbose, a.u.sieve_length, m, is_last, it->second.ul, it->second.uu, prev_p, next_p,
+                          r.p_tests, r.n_tests, merit);
+            done.erase(d);
+            pending.erase(it);
+        }
+        (void)all;
+    };
+    auto drain = [&](bool wait) -> bool {
+        size_t n = 0;
+        if (pgap_scan_poll(e, rows.data(), rows.size(), &n, wait ? 1 : 0) != PGAP_OK) {
+            fprintf(stderr, "scan: %s\n", pgap_last_error(e));
+            return false;
+        }
+        for (size_t i = 0; i < n; ++i) done[rows[i].m] = rows[i];
+        flush(false);
+        return true;
+    };
+
+    std::string line, err;
+    std::vector<int32_t> prev, next;
+    for (uint64_t mi = 0; mi < a.u.minc && status == 0; ++mi) {
+        const uint64_t m = a.u.mstart + mi;
+        if (gcd64(m, a.u.d) > 1) continue;
+        if (!std::getline(in, line)) {
+            fprintf(stderr, "unknown file ended before m=%lu\n", (unsigned long)m);
+            status = 1;
+            break;
+        }
+        if (m < a.mskip) continue;
+        uint64_t mp = 0;
+        if (!parse_unknown_line(line, format, format == 2 ? &bl : nullptr, mp, prev, next, err) || mp != m) {
+            fprintf(stderr, "bad line for m=%lu: %s\n", (unsigned long)m, mp != m ? "m mismatch" : err.c_str());
+            status = 1;
+            break;
+        }
+        if (a.dump) {
+            printf("%s", plain_line(m, prev, next).c_str());
+            continue;
+        }
+        pgap_scan_req req{};
+        req.family = fam;
+        req.m = m;
+        req.prev = prev.data();
+        req.n_prev = (uint32_t)prev.size();
+        req.next = next.data();
+        req.n_next = (uint32_t)next.size();
+        req.tag = m;
+        for (;;) {
+            pgap_status st = pgap_scan_submit(e, &req);
+            if (st == PGAP_OK) break;
+            if (st != PGAP_E_OOM) {
+                fprintf(stderr, "scan submit m=%lu: %s\n", (unsigned long)m, pgap_last_error(e));
+                status = 1;
+                break;
+            }
+            if (!drain(true)) {
+                status = 1;
+                break;
+            }
+        }
+        if (status) break;
+        pending[m] = Pending{prev.size(), next.size()};
+    }
+    while (status == 0 && e && pgap_scan_in_flight(e) > 0)
+        if (!drain(true)) status = 1;
+    if (status == 0) flush(true);
+    if (results) fclose(results);
+    pgap_destroy(e);
+    return status;
+}

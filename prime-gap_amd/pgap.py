@@ -36,6 +36,7 @@ EXPORTED = (
     "pgap_prp_batch", "pgap_prp_batch_device", "pgap_prp_batch_mixed", "pgap_sync",
     "pgap_last_launch", "pgap_family", "pgap_scan_submit", "pgap_scan_poll",
     "pgap_scan_in_flight", "pgap_scan_config", "pgap_build_candidates_device", "pgap_filter_candidates",
+    "pgap_sieve",
 )
 
 
@@ -125,6 +126,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.pgap_scan_config.argtypes = [vp, u32, u32]
     lib.pgap_build_candidates_device.argtypes = [vp, ctypes.c_int, vp, vp, sz, u32, vp]
     lib.pgap_filter_candidates.argtypes = [vp, ctypes.c_int, vp, vp, sz, u32, vp]
+    lib.pgap_sieve.argtypes = [vp, ctypes.c_int, vp, sz, u64, vp, vp, vp, vp]
     for name in EXPORTED:
         f = getattr(lib, name)
         if name not in ("pgap_destroy", "pgap_last_error", "pgap_version", "pgap_scan_in_flight"):
@@ -257,6 +259,18 @@ class Engine:
         self._check(self.lib.pgap_filter_candidates(self.h, family, mm.ctypes.data, xx.ctypes.data, len(mm), limit,
                                                     keep.ctypes.data))
         return keep
+
+    def sieve(self, family: int, ms, max_prime: int, sieve_length: int):
+        """-> list of (prev distances, next distances) per m (device sieve)."""
+        mm = np.ascontiguousarray(ms, dtype=np.uint64)
+        n = len(mm)
+        prev = np.zeros((n, sieve_length), dtype=np.int32)
+        nxt = np.zeros((n, sieve_length), dtype=np.int32)
+        npv = np.zeros(n, dtype=np.uint32)
+        nnx = np.zeros(n, dtype=np.uint32)
+        self._check(self.lib.pgap_sieve(self.h, family, mm.ctypes.data, n, max_prime, prev.ctypes.data,
+                                        npv.ctypes.data, nxt.ctypes.data, nnx.ctypes.data))
+        return [(prev[i, : npv[i]].copy(), nxt[i, : nnx[i]].copy()) for i in range(n)]
 
     def scan_config(self, depth: int, confirm_rounds: int):
         self._check(self.lib.pgap_scan_config(self.h, depth, confirm_rounds))

@@ -1,0 +1,90 @@
+"""gap_test_hip end to end on the GPU: rows and stats lines equal what
+gap_test_simple prints (gap_test_simple.cpp:101-200, gap_test_common.cpp:59-124)
+for the oracle's per-m results, after the reference's own normalisation of
+timing figures (misc/tests.sh:119-120)."""
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import unknown_format as uf
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(REPO, "prime-gap_amd", "gap_test_hip")
+FIXTURE = os.path.join(REPO, "tests", "golden", "907_2190_1_200_s11000_l100M.txt")
+
+
+def normalise(text: str) -> str:
+    # misc/tests.sh:119-120
+    text = re.sub(r"\([0-9.]+ secs/test\)", "(XX.YY/sec)", text)  # a slow first m prints secs/test
+    text = re.sub(r"[0-9.]+( ?)(tests)?/sec\)", r"XX.YY\1\2/sec)", text)
+    text = re.sub(r"[0-9]+ sec", "Z sec", text)
+    return re.sub(r"Reading unknowns from '[^']*'", "Reading unknowns from 'FILE'", text)
+
+
+def expected_stdout(P, D, ms, mi, SL, lines, min_merit, verbose):
+    K_log = ol.K_log(P, D)
+    out = []
+    w = out.append
+    mm = np.float32(min_merit)
+    w("\n")
+    w(f"Testing m * {P}#/{D}, m = {ms} + [0, {mi})\n")
+    w("\nReading unknowns from 'FILE'\n")
+    w(f"Min Gap ~= {int(float(mm) * (K_log + float(np.float32(math.log(ms)))))} (for merit > {float(mm):.1f})\n")
+    valid = [m for m in range(ms, ms + mi) if math.gcd(m, D) == 1]
+    last_mi = max(m - ms for m in valid)
+    w(f"\n{len(valid)} tests M_start({ms}) + mi(0 to {last_mi})\n\n")
+    s_tests = tu = tp = tn = prp = 0
+    for ln in lines:
+        m, p, n = uf.parse_line(ln)
+        r = ol.test_interval(P, D, m, SL, p, n, kind=0)
+        gap = r["prev_p"] + r["next_p"]
+        merit = np.float32(gap / (K_log + math.log(m)))
+        if merit > mm:
+            w("%-5d %.4f  %d * %d#/%d -%d to +%d\n" % (gap, merit, m, P, D, r["prev_p"], r["next_p"]))
+        s_tests += 1
+        tu += len(p) + len(n)
+        tp += len(p)
+        tn += len(n)
+        prp += r["tests"]
+        is_last = (m - ms) == last_mi
+        power = s_tests == 1
+        q = 10
+        while q <= s_tests:
+            power |= s_tests in (q, 3 * q, 5 * q)
+            q *= 10
+        if (is_last or power) and verbose + is_last >= 1:
+            w("\tm=%d %4d <- unknowns -> %-4d\t%4d <- gap -> %-4d\n" % (m, len(p), len(n), r["prev_p"], r["next_p"]))
+            w("\t    tests     %-10d (XX.YY/sec)  Z seconds elapsed\n" % s_tests)
+            w("\t    unknowns  %-10d (avg: %.2f), %.2f%% composite  %.2f%% <- %% -> %.2f%%\n" % (
+                tu, tu / s_tests, 100.0 * (1 - tu / ((2.0 * SL + 1) * s_tests)), 100.0 * tp / tu, 100.0 * tn / tu))
+            w("\t    prp tests %-10d (avg: %.2f) (XX.YY tests/sec)\n" % (prp, np.float32(prp / np.float32(s_tests))))
+    return "".join(out)
+
+
+def test_cli_matches_gap_test_simple_output():
+    r = subprocess.run([CLI, "-u", FIXTURE, "--min-merit", "8", "-q"], capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stderr.decode()
+    lines = open(FIXTURE, "rb").read().splitlines(keepends=True)
+    want = expected_stdout(907, 2190, 1, 200, 11000, lines, 8.0, 1)
+    assert normalise(r.stdout.decode()) == want
+
+
+def test_cli_results_file_and_mskip(tmp_path):
+    res = tmp_path / "results.tsv"
+    r = subprocess.run([CLI, "-u", FIXTURE, "--min-merit", "8", "-qq", "--mskip", "100", "--depth", "2",
+                        "--results", str(res)], capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stderr.decode()
+    rows = [l.split("\t") for l in res.read_text().splitlines()]
+    lines = {uf.parse_line(ln)[0]: uf.parse_line(ln) for ln in open(FIXTURE, "rb").read().splitlines(keepends=True)}
+    assert [int(x[2]) for x in rows] == [m for m in sorted(lines) if m >= 100]
+    for P, D, m, next_p, prev_p, merit in rows:
+        _, p, n = lines[int(m)]
+        want = ol.test_interval(907, 2190, int(m), 11000, p, n, kind=0)
+        assert (int(prev_p), int(next_p)) == (want["prev_p"], want["next_p"])
