@@ -71,6 +71,133 @@ def cpu_baseline(numbers_host, sample, threads):
     return arr.shape[0] / dt, dt, verdicts
 
 
+def primorial_over(P, D):
+    return math.prod(p for p in range(2, P + 1) if all(p % q for q in range(2, int(p ** 0.5) + 1))) // D
+
+
+def bench_batch_2180(engine, hipmem, pgap, count, steps, seed):
+    """The 2180-bit width of the metric: N = m*1511#/2190 + x, m ~ U[1, 1e6)
+    coprime to 2190, x ~ U[-20000, 20000], no prime factor <= 1e6."""
+    P2, D2 = 1511, 2190
+    fam = engine.family(P2, D2, 0)
+    rng = np.random.default_rng(seed)
+    ms, xs, have = [], [], 0
+    while have < count:
+        m = rng.integers(1, 1_000_000, size=1 << 20, dtype=np.int64)
+        m = m[np.gcd(m, D2) == 1]
+        x = rng.integers(-20000, 20001, size=len(m), dtype=np.int64)
+        keep = engine.filter_candidates(fam, m.astype(np.uint64), x, SIEVE_LIMIT).astype(bool)
+        ms.append(m[keep])
+        xs.append(x[keep])
+        have += int(keep.sum())
+    m = np.concatenate(ms)[:count].astype(np.uint64)
+    x = np.concatenate(xs)[:count]
+    K = primorial_over(P2, D2)
+    bits_hi = (K * 1_000_000 + 20000).bit_length()
+    limbs = (bits_hi + 31) // 32
+    d_n = hipmem.DeviceArray((count, limbs), np.uint32)
+    d_out = hipmem.DeviceArray(count, np.uint8)
+    engine.build_candidates_device(fam, m, x, limbs, d_n.ptr)
+    e_mean = float(np.mean([(int(a) * K + int(b)).bit_length() for a, b in zip(m[:2048], x[:2048])]))
+    ks = []
+    for i in range(steps + 1):
+        engine.prp_device(d_n.ptr, limbs, count, d_out.ptr, pgap.FERMAT2, (), bits_hi)
+        engine.sync()
+        if i:
+            ks.append(engine.last_launch().kernel_ms)
+    info = engine.last_launch()
+    avg = float(np.mean(ks)) * 1e-3
+    L32 = (int(e_mean + 0.999) + 31) // 32
+    return {
+        "workload": f"N = m*{P2}#/{D2} + x, m~U[1,1e6) coprime to d, x~U[-20000,20000], no factor <= 1e6, "
+                    f"{count} candidates, {e_mean:.0f} bits",
+        "tests_per_s": count / avg,
+        "kernel_ms_avg": avg * 1e3,
+        "kernel": f"radix-2^{info.radix_bits} x {info.digits} digits",
+        "roofline_frac": 2 * L32 * L32 * e_mean * count / avg / PEAK_INT32_MAC,
+        "work_per_test": f"2*L^2*e, L={L32}, e={e_mean:.1f}",
+    }
+
+
+def bench_stream(engine, pgap, n_m, load, seed, cpu_sample, threads):
+    """Serving: P=1511, d=2190, SL=40000, m's in seeded random order from
+    [1, 200000) coprime to d, each with its unknowns sieved to 1e9 (device
+    sieve, untimed).  Phase 1 saturates the engine (capacity); phase 2 feeds
+    Poisson arrivals at `load` x capacity and records arrival-to-row latency."""
+    P4, D4, SL4 = 1511, 2190, 40000
+    fam = engine.family(P4, D4, SL4)
+    rng = np.random.default_rng(seed)
+    pool = np.array([m for m in range(1, 200000) if math.gcd(m, D4) == 1], dtype=np.uint64)
+    order = rng.permutation(pool)[: 2 * n_m]
+    t0 = time.perf_counter()
+    lists = engine.sieve(fam, order, 1_000_000_000, SL4)
+    sieve_s = time.perf_counter() - t0
+    cap_set, lat_set = list(range(n_m)), list(range(n_m, 2 * n_m))
+
+    def run(idx, arrivals):
+        rows = {}
+        pending = list(idx)
+        pos = 0
+        start = time.monotonic()
+        arr_abs = {}
+        while pos < len(pending) or engine.scan_in_flight():
+            now = time.monotonic()
+            while pos < len(pending):
+                i = pending[pos]
+                if arrivals is not None and start + arrivals[pos] > now:
+                    break
+                p, q = lists[i]
+                if engine.scan_submit(fam, int(order[i]), p, q, tag=i) == pgap.PGAP_E_OOM:
+                    break
+                arr_abs[i] = start + arrivals[pos] if arrivals is not None else now
+                pos += 1
+            if engine.scan_in_flight():
+                for r in engine.scan_poll(wait=False):
+                    rows[r.tag] = (r, r.t_done - arr_abs[r.tag])
+            elif pos < len(pending):
+                time.sleep(max(0.0, start + arrivals[pos] - time.monotonic()))
+        return rows, time.monotonic() - start
+
+    s0 = engine.scan_stats()
+    rows_cap, cap_s = run(cap_set, None)
+    s1 = engine.scan_stats()
+    useful = sum(r.p_tests + r.n_tests for r, _ in rows_cap.values())
+    cap_mps = len(rows_cap) / cap_s
+    gaps = np.cumsum(rng.exponential(1.0 / (load * cap_mps), size=len(lat_set)))
+    rows_lat, lat_s = run(lat_set, gaps)
+    lat_ms = np.array([lat for _, lat in rows_lat.values()]) * 1e3
+    out = {
+        "workload": f"P={P4} d={D4} SL={SL4}, m in seeded random order from [1,200000) coprime to d, unknowns "
+                    f"sieved to 1e9 ({np.mean([len(p) + len(q) for p, q in lists]):.0f} per m), Fermat-2 scan "
+                    f"with 2 strong-PRP confirmations, depth <= 8",
+        "m": n_m,
+        "capacity_m_per_s": cap_mps,
+        "useful_tests_per_s": useful / cap_s,
+        "issued_tests_per_s": (s1["tests_issued"] - s0["tests_issued"]) / cap_s,
+        "offered_load": load,
+        "p50_ms": float(np.percentile(lat_ms, 50)),
+        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "latency_phase_s": lat_s,
+        "sieve_s_untimed": sieve_s,
+        "slots": s1["slots"],
+    }
+    if cpu_sample:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import oracle_lib
+        ids = cap_set[:cpu_sample]
+        t0 = time.perf_counter()
+        rows = oracle_lib.scan_many(P4, D4, SL4, [int(order[i]) for i in ids], [lists[i] for i in ids], 0, threads)
+        dt = time.perf_counter() - t0
+        mism = sum(1 for k, i in enumerate(ids)
+                   if (rows_cap[i][0].prev_p, rows_cap[i][0].next_p) != (rows[k][0], rows[k][1]))
+        out["cpu_baseline"] = {"m_per_s": len(ids) / dt, "tests_per_s": float(rows[:, 2].sum()) / dt,
+                               "cores": threads, "kind": "port",
+                               "sample": f"{len(ids)} of the same m's, gap_test_simple semantics "
+                                         f"(mpz_probab_prime_p(N,25) scan), {threads} threads, {dt:.1f} s"}
+        out["parity_sample_mismatches"] = mism
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,7 +207,24 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202 + 1)
     ap.add_argument("--cpu-sample", type=int, default=32768)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-2180", action="store_true", help="skip the 2180-bit batch")
+    ap.add_argument("--no-stream", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--stream-m", type=int, default=8192)
+    ap.add_argument("--stream-load", type=float, default=0.5)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 3 times (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import hipmem
+        n = args.copy_calibration << 20
+        a = hipmem.DeviceArray(n, np.uint8).zero()
+        b = hipmem.DeviceArray(n, np.uint8)
+        for _ in range(3):
+            b.copy_from(a)
+        hipmem.synchronize()
+        print(json.dumps({"copy_bytes_per_dispatch": n}), flush=True)
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -184,6 +328,15 @@ def main():
                       f"{threads} threads, {dt:.1f} s",
         }
         result["parity_sample_mismatches"] = mism
+    extra = {}
+    if not args.no_2180:
+        extra["batch_2180_bit"] = bench_batch_2180(engine, hipmem, pgap, 1 << 18, 3, args.seed + 1)
+    if not args.no_stream:
+        threads = min(16, os.cpu_count() or 1)
+        extra["stream"] = bench_stream(engine, pgap, args.stream_m, args.stream_load, args.seed + 2,
+                                       0 if args.no_cpu else 64, threads)
+    if extra:
+        result["extra"] = extra
     print(json.dumps(result), flush=True)
     engine.close()
     return 0

@@ -171,6 +171,11 @@ pgap_status pgap_scan_poll(pgap_engine *e, pgap_scan_row *rows, size_t cap, size
 /* Number of m's submitted and not yet returned. */
 size_t pgap_scan_in_flight(const pgap_engine *e);
 
+/* Scan telemetry since the first submit: PRP tests issued (including the
+ * speculative ones past a side's first PRP), rounds run, and the number of
+ * slots the HBM budget allows. */
+pgap_status pgap_scan_stats(const pgap_engine *e, uint64_t *tests_issued, uint64_t *rounds, uint32_t *slots);
+
 /* Streaming knobs: speculative depth per side (tests in flight per side),
  * Miller-Rabin confirmation rounds on each side's first PRP (bases 3, 5, ...),
  * and the confirmation's effect: a found PRP failing confirmation is treated

@@ -348,3 +348,52 @@ int or_sieve_m(const or_sieve *s, uint64_t m, int32_t *prev, int *np, int32_t *n
     free(comp);
     return 0;
 }
+
+/* ---- threaded gap_test_simple baseline ---- */
+typedef struct {
+    uint32_t P, D, SL;
+    const uint64_t *m;
+    const int32_t *prev, *next;
+    const uint64_t *prev_off, *next_off;
+    const uint32_t *n_prev, *n_next;
+    int kind;
+    size_t lo, hi;
+    int64_t *rows;
+} scan_job;
+
+static void *scan_worker(void *arg) {
+    scan_job *j = (scan_job *)arg;
+    int64_t out[7];
+    for (size_t i = j->lo; i < j->hi; i++) {
+        or_test_interval(j->P, j->D, j->m[i], j->SL, j->prev + j->prev_off[i], (int)j->n_prev[i],
+                         j->next + j->next_off[i], (int)j->n_next[i], j->kind, out);
+        if (j->rows) {
+            j->rows[3 * i] = out[0];
+            j->rows[3 * i + 1] = out[1];
+            j->rows[3 * i + 2] = out[2];
+        }
+    }
+    return NULL;
+}
+
+size_t or_bench_scan(uint32_t P, uint32_t D, uint32_t SL, const uint64_t *m, size_t count, const int32_t *prev,
+                     const uint64_t *prev_off, const uint32_t *n_prev, const int32_t *next,
+                     const uint64_t *next_off, const uint32_t *n_next, int kind, int threads, int64_t *rows) {
+    if (threads <= 0) threads = 1;
+    if ((size_t)threads > count) threads = count ? (int)count : 1;
+    pthread_t *tid = (pthread_t *)calloc((size_t)threads, sizeof(pthread_t));
+    scan_job *jobs = (scan_job *)calloc((size_t)threads, sizeof(scan_job));
+    for (int t = 0; t < threads; t++) {
+        scan_job *j = &jobs[t];
+        j->P = P; j->D = D; j->SL = SL; j->m = m; j->prev = prev; j->next = next;
+        j->prev_off = prev_off; j->next_off = next_off; j->n_prev = n_prev; j->n_next = n_next;
+        j->kind = kind; j->rows = rows;
+        j->lo = count * (size_t)t / (size_t)threads;
+        j->hi = count * (size_t)(t + 1) / (size_t)threads;
+        pthread_create(&tid[t], NULL, scan_worker, j);
+    }
+    for (int t = 0; t < threads; t++) pthread_join(tid[t], NULL);
+    free(tid);
+    free(jobs);
+    return count;
+}

@@ -86,6 +86,14 @@ size_t or_sieve_nprimes(const or_sieve *s);
  * pthreads.  Return the number of items done.  kind as in or_verdicts. */
 size_t or_bench_verdicts(const uint32_t *n, size_t count, int limbs, int kind, int threads);
 
+/* gap_test_simple over count m's (gap_test_simple.cpp:152-195 calling
+ * test_interval_cpu), m's split statically over `threads` pthreads.  List i
+ * is prev[prev_off[i] .. prev_off[i] + n_prev[i]) (likewise next).  kind as
+ * in or_test_interval.  rows[3*i..] = prev_p, next_p, prp tests. */
+size_t or_bench_scan(uint32_t P, uint32_t D, uint32_t SL, const uint64_t *m, size_t count, const int32_t *prev,
+                     const uint64_t *prev_off, const uint32_t *n_prev, const int32_t *next,
+                     const uint64_t *next_off, const uint32_t *n_next, int kind, int threads, int64_t *rows);
+
 #ifdef __cplusplus
 }
 #endif

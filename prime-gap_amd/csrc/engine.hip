@@ -223,6 +223,9 @@ pgap_status pgap_create(const pgap_options *opt, pgap_engine **out) {
         *out = e;
         return PGAP_E_DEVICE;
     }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->opt.device) == hipSuccess && cus > 0)
+        e->num_cus = cus;
     e->ready = true;
     *out = e;
     return PGAP_OK;

@@ -42,6 +42,7 @@ struct Family {
 struct pgap_engine {
     pgap_options opt{};
     bool ready = false;
+    int num_cus = 256;  // compute units of the device (queried at create)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timing_pending = false;

@@ -35,6 +35,7 @@ struct Variant {
     int kind;        // Kind
     int max_bits;    // largest N the variant accepts (lazy reduction needs N < R/16)
     batch_launch_fn launch;
+    int (*blocks_per_cu)();  // resident 256-thread workgroups per CU (occupancy query)
 };
 
 // Registry: filled by every instantiation unit at library load.

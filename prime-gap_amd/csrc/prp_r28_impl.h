@@ -40,11 +40,23 @@ hipError_t launch_r28(const uint32_t *d_n, uint32_t limbs, uint32_t count, const
     return hipGetLastError();
 }
 
+template <int L, int KIND>
+int occupancy_r28() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, prp_r28_kernel<L, KIND>, 256, 0) != hipSuccess || n <= 0)
+        n = 1;
+    return n;
+}
+
 template <int... Ls>
 struct RegisterR28 {
     RegisterR28() {
-        (register_variant(Variant{28, Ls, 1, KIND_FERMAT2, 28 * Ls - 4, &launch_r28<Ls, KIND_FERMAT2>}), ...);
-        (register_variant(Variant{28, Ls, 1, KIND_SPRP, 28 * Ls - 14, &launch_r28<Ls, KIND_SPRP>}), ...);
+        (register_variant(Variant{28, Ls, 1, KIND_FERMAT2, 28 * Ls - 4, &launch_r28<Ls, KIND_FERMAT2>,
+                                  &occupancy_r28<Ls, KIND_FERMAT2>}),
+         ...);
+        (register_variant(
+             Variant{28, Ls, 1, KIND_SPRP, 28 * Ls - 14, &launch_r28<Ls, KIND_SPRP>, &occupancy_r28<Ls, KIND_SPRP>}),
+         ...);
     }
 };
 

@@ -37,7 +37,7 @@ namespace pgap {
 constexpr uint32_t NONE = 0xffffffffu;
 constexpr uint32_t FALLBACK_CHUNK = 4096;  // candidates appended per exhausted side
 constexpr uint32_t FALLBACK_SPAN = 65536;  // offsets screened per append
-constexpr uint32_t MAX_SLOTS = 1u << 16;
+constexpr uint32_t MAX_SLOTS = 1u << 14;  // enough m's in flight to fill the device at depth 2
 
 enum : uint32_t {
     F_PREV_DONE = 1,
@@ -239,7 +239,7 @@ struct Staged {
 class Scanner {
   public:
     pgap_engine *e;
-    uint32_t depth = 8, confirm_rounds = 2;
+    uint32_t depth = 32, confirm_rounds = 2;
     uint32_t S = 0, CW = 0, CAP = 0, W = 0, MAXI = 0;
     bool inited = false;
 
@@ -513,16 +513,22 @@ class Scanner {
         if ((st = admit())) return st;
         if (active.empty()) return PGAP_OK;
         const uint32_t na = (uint32_t)active.size();
-        // speculative depth: fill about one wave per SIMD when few sides are
-        // active, one test per side... up to `depth` when the device is busy
-        const uint32_t target = 65536;
-        uint32_t d = std::max<uint32_t>(1, std::min<uint32_t>(depth_max(), (target + 2 * na - 1) / (2 * na)));
         uint32_t bits = 0;
         for (uint32_t s : active) bits = std::max(bits, slots[s].bits);
         bits += 1;  // + x
         const Variant *vf, *vc = nullptr;
         if ((st = e->pick(KIND_FERMAT2, bits, &vf))) return st;
         if (confirm_rounds && (st = e->pick(KIND_SPRP, bits, &vc))) return st;
+        // Speculative depth.  A round costs about one test's latency whether
+        // it carries one item or a full device's worth (the kernel is VALU
+        // bound per lane), so every round is sized to the numbers the chosen
+        // variant keeps resident at once: few active sides get a deep
+        // speculative window, many get one or two tests each.
+        const uint32_t resident =
+            (uint32_t)std::max(1, vf->blocks_per_cu ? vf->blocks_per_cu() : 1) * e->num_cus * 256u /
+            (uint32_t)std::max(1, vf->lanes);
+        const uint32_t d =
+            std::max<uint32_t>(1, std::min<uint32_t>(depth_max(), (resident + 2 * na - 1) / (2 * na)));
         Bases none{}, conf{};
         static const uint32_t mr_bases[kMaxBases] = {3, 5, 7, 11, 13, 17, 19, 23};
         conf.n = std::min<uint32_t>(confirm_rounds, kMaxBases);
@@ -731,11 +737,13 @@ size_t pgap_scan_in_flight(const pgap_engine *e) {
     return e->scanner->in_flight();
 }
 
-pgap_status pgap_prp_batch_mixed(pgap_engine *e, const uint32_t *, const uint64_t *, const uint32_t *, size_t,
-                                 pgap_test, const uint32_t *, uint32_t, uint8_t *) {
+pgap_status pgap_scan_stats(const pgap_engine *e, uint64_t *tests_issued, uint64_t *rounds, uint32_t *slots) {
     if (!e) return PGAP_E_ARG;
-    e->set_error("mixed batches: not built yet");
-    return PGAP_E_STATE;
+    const Scanner *s = e->scanner;
+    if (tests_issued) *tests_issued = s ? s->tests_issued : 0;
+    if (rounds) *rounds = s ? s->rounds : 0;
+    if (slots) *slots = s ? s->S : 0;
+    return PGAP_OK;
 }
 
 }  // extern "C"

@@ -61,6 +61,11 @@ class DeviceArray:
         _ck(hip().hipMemcpy(d.ptr, a.ctypes.data, a.nbytes, H2D), "hipMemcpy H2D")
         return d
 
+    def copy_from(self, other: "DeviceArray"):
+        n = min(self.nbytes, other.nbytes)
+        _ck(hip().hipMemcpy(self.ptr, other.ptr, n, D2D), "hipMemcpy D2D")
+        return self
+
     def zero(self):
         _ck(hip().hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
         return self

@@ -36,7 +36,7 @@ EXPORTED = (
     "pgap_prp_batch", "pgap_prp_batch_device", "pgap_prp_batch_mixed", "pgap_sync",
     "pgap_last_launch", "pgap_family", "pgap_scan_submit", "pgap_scan_poll",
     "pgap_scan_in_flight", "pgap_scan_config", "pgap_build_candidates_device", "pgap_filter_candidates",
-    "pgap_sieve",
+    "pgap_sieve", "pgap_scan_stats",
 )
 
 
@@ -127,6 +127,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.pgap_build_candidates_device.argtypes = [vp, ctypes.c_int, vp, vp, sz, u32, vp]
     lib.pgap_filter_candidates.argtypes = [vp, ctypes.c_int, vp, vp, sz, u32, vp]
     lib.pgap_sieve.argtypes = [vp, ctypes.c_int, vp, sz, u64, vp, vp, vp, vp]
+    lib.pgap_scan_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u32)]
     for name in EXPORTED:
         f = getattr(lib, name)
         if name not in ("pgap_destroy", "pgap_last_error", "pgap_version", "pgap_scan_in_flight"):
@@ -294,3 +295,8 @@ class Engine:
 
     def scan_in_flight(self) -> int:
         return self.lib.pgap_scan_in_flight(self.h)
+
+    def scan_stats(self) -> dict:
+        t, r, s = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint32()
+        self._check(self.lib.pgap_scan_stats(self.h, ctypes.byref(t), ctypes.byref(r), ctypes.byref(s)))
+        return {"tests_issued": t.value, "rounds": r.value, "slots": s.value}

@@ -46,6 +46,8 @@ def load():
     lib.or_sieve_nprimes.restype = sz
     lib.or_bench_verdicts.argtypes = [vp, sz, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.or_bench_verdicts.restype = sz
+    lib.or_bench_scan.argtypes = [u32, u32, u32, vp, sz, vp, vp, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
+    lib.or_bench_scan.restype = sz
     _lib = lib
     return lib
 
@@ -106,6 +108,25 @@ def test_interval(P, D, m, SL, prev, nxt, kind=0) -> dict:
     load().or_test_interval(P, D, m, SL, p.ctypes.data, len(p), q.ctypes.data, len(q), kind, out.ctypes.data)
     keys = ("prev_p", "next_p", "tests", "prev_fallback", "next_fallback", "p_tests", "n_tests")
     return dict(zip(keys, (int(v) for v in out)))
+
+
+def scan_many(P, D, SL, ms, lists, kind=0, threads=8) -> np.ndarray:
+    """gap_test_simple over many m's on `threads` threads -> rows [n][3]
+    (prev_p, next_p, prp tests)."""
+    n = len(ms)
+    m = np.ascontiguousarray(ms, dtype=np.uint64)
+    npv = np.array([len(p) for p, _ in lists], dtype=np.uint32)
+    nnx = np.array([len(q) for _, q in lists], dtype=np.uint32)
+    poff = np.zeros(n, dtype=np.uint64)
+    qoff = np.zeros(n, dtype=np.uint64)
+    poff[1:] = np.cumsum(npv[:-1], dtype=np.uint64)
+    qoff[1:] = np.cumsum(nnx[:-1], dtype=np.uint64)
+    pv = np.concatenate([np.asarray(p, np.int32) for p, _ in lists] + [np.zeros(1, np.int32)])
+    nx = np.concatenate([np.asarray(q, np.int32) for _, q in lists] + [np.zeros(1, np.int32)])
+    rows = np.zeros((n, 3), dtype=np.int64)
+    load().or_bench_scan(P, D, SL, m.ctypes.data, n, pv.ctypes.data, poff.ctypes.data, npv.ctypes.data,
+                         nx.ctypes.data, qoff.ctypes.data, nnx.ctypes.data, kind, threads, rows.ctypes.data)
+    return rows
 
 
 class Sieve:

@@ -94,6 +94,46 @@ def test_large_family_and_sprp(engine):
     assert engine.prp_ints([m2203])[0] == 1
 
 
+@pytest.mark.parametrize("bits", [2300, 2490, 2600, 3000, 3320, 3500, 4100, 4250])
+def test_quad_variants_random_sizes(engine, bits):
+    # wider than one lane holds: four lanes per number
+    rng = random.Random(bits)
+    vals = [rng.getrandbits(bits) | (1 << (bits - 1)) | 1 for _ in range(96)]
+    p = []
+    for v in vals[:2]:
+        x = v
+        while not ol.probab_prime(x):
+            x += 2
+        p.append(x)
+    got = check(engine, vals + p)
+    assert got[-2:].all()
+    assert engine.last_launch().lanes_per_number == 4
+
+
+def test_quad_variants_special_and_sprp(engine):
+    vals = [(1 << 2281) - 1, (1 << 3217) - 1, (1 << 4253) - 1, (1 << 4096) + 1, (1 << 2203) * 3 + 1,
+            (1 << 3001) - 1]
+    got = check(engine, vals)
+    assert list(got[:4]) == [1, 1, 1, 1]
+    strong = [v for v in vals if v.bit_length() <= 26 * 164 - 14]  # widest strong-PRP variant
+    check(engine, strong, pgap.SPRP, (2,))
+    check(engine, strong, pgap.SPRP, (3, 5))
+    c = candidates(3001, 2190, 1, 100_000, 40000, 64, 17)   # 4231-4248-bit family
+    check(engine, c)
+    check(engine, c[:16], pgap.SPRP, (3, 5))
+
+
+@pytest.mark.parametrize("bits", [100, 700, 1108, 2000])
+def test_quad_variants_forced_small(bits):
+    # the quad kernels checked where the one-lane kernels also work
+    rng = random.Random(bits + 7)
+    vals = [rng.getrandbits(bits) | (1 << (bits - 1)) | 1 for _ in range(200)] + [341, 561, 2047, (1 << 521) - 1]
+    with pgap.Engine(0, lanes=4) as e:
+        check(e, vals)
+        assert e.last_launch().lanes_per_number == 4
+        check(e, vals[:60], pgap.SPRP, (2, 3))
+
+
 def test_rejects_what_no_variant_covers(engine):
     with pytest.raises(pgap.PgapError) as ei:
         engine.prp_ints([(1 << 9000) + 1])

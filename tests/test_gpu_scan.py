@@ -118,6 +118,21 @@ def test_scan_2100_bit_family_small():
             (want["prev_p"], want["next_p"], want["p_tests"], want["n_tests"]), m
 
 
+def test_scan_4300_bit_family_small():
+    # 3001#/2190 (N ~ 4231-4248 bits): quad-per-number variants in the scan
+    P, D, SL = 3001, 2190, 40000
+    sv = ol.Sieve(P, D, SL, 1_000_000)
+    reqs = [(m,) + sv(m) for m in (1, 7)]
+    with pgap.Engine(0) as e:
+        fam = e.family(P, D, SL)
+        rows = run_scan(e, fam, reqs)
+    for m, p, n in reqs:
+        want = ol.test_interval(P, D, m, SL, p, n, kind=1)
+        r = rows[m]
+        assert (r.prev_p, r.next_p, r.p_tests, r.n_tests) == \
+            (want["prev_p"], want["next_p"], want["p_tests"], want["n_tests"]), m
+
+
 def test_budget_and_admission(unknowns_907):
     lines = lines_907(unknowns_907)
     with pgap.Engine(0, hbm_budget=1000) as e:
