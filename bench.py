@@ -119,6 +119,129 @@ def bench_batch_2180(engine, hipmem, pgap, count, steps, seed):
     }
 
 
+def sieved_family_rows(engine, hipmem, P2, D2, mlo, mhi, xr, count, rng):
+    """count rows N = m*K + x (no factor <= 1e6) as a host [count][limbs] array."""
+    fam = engine.family(P2, D2, 0)
+    ms, xs, have = [], [], 0
+    while have < count:
+        m = rng.integers(mlo, mhi, size=1 << 20, dtype=np.int64)
+        m = m[np.gcd(m, D2) == 1]
+        x = rng.integers(-xr, xr + 1, size=len(m), dtype=np.int64)
+        keep = engine.filter_candidates(fam, m.astype(np.uint64), x, SIEVE_LIMIT).astype(bool)
+        ms.append(m[keep])
+        xs.append(x[keep])
+        have += int(keep.sum())
+    m = np.concatenate(ms)[:count].astype(np.uint64)
+    x = np.concatenate(xs)[:count]
+    K = primorial_over(P2, D2)
+    bits_hi = (K * (mhi - 1) + xr).bit_length()
+    limbs = (bits_hi + 31) // 32
+    d = hipmem.DeviceArray((count, limbs), np.uint32)
+    engine.build_candidates_device(fam, m, x, limbs, d.ptr)
+    rows = d.to_host()
+    d.free()
+    e_mean = float(np.mean([(int(a) * K + int(b)).bit_length() for a, b in zip(m[:1024], x[:1024])]))
+    return rows, bits_hi, e_mean
+
+
+def bench_mixed(engine, hipmem, pgap, count, seed):
+    """Uneven batching: 40% 503#/210, 40% 1511#/2190, 20% 809#/1841070,
+    m ~ U[1, 1e6) coprime to d, x ~ U[-20000, 20000], no factor <= 1e6, in a
+    seeded shuffle.  Compared: one persistent launch over all widths, one
+    launch per width, and every number in the widest width (one container)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for (P2, D2, frac) in ((503, 210, 0.4), (1511, 2190, 0.4), (809, 1841070, 0.2)):
+        n = int(round(count * frac))
+        rows, bits_hi, e_mean = sieved_family_rows(engine, hipmem, P2, D2, 1, 1_000_000, 20000, n, rng)
+        parts.append((P2, rows, bits_hi, e_mean))
+    total = sum(len(r) for _, r, _, _ in parts)
+    # packed, shuffled
+    fam_of = np.concatenate([np.full(len(r), i) for i, (_, r, _, _) in enumerate(parts)])
+    idx_in = np.concatenate([np.arange(len(r)) for _, r, _, _ in parts])
+    perm = rng.permutation(total)
+    fam_of, idx_in = fam_of[perm], idx_in[perm]
+    limbs = np.array([parts[f][1].shape[1] for f in fam_of], dtype=np.uint32)
+    offset = np.zeros(total, dtype=np.uint64)
+    offset[1:] = np.cumsum(limbs[:-1], dtype=np.uint64)
+    words = np.zeros(int(limbs.sum()), dtype=np.uint32)
+    for f, (_, rows, _, _) in enumerate(parts):
+        sel = np.nonzero(fam_of == f)[0]
+        w = rows.shape[1]
+        pos = (offset[sel][:, None] + np.arange(w, dtype=np.uint64)[None, :]).ravel()
+        words[pos] = rows[idx_in[sel]].ravel()
+    engine.prp_mixed_packed(words, offset, limbs)          # warm-up
+    v_mixed = engine.prp_mixed_packed(words, offset, limbs)
+    t_mixed = engine.last_launch().kernel_ms
+    # one launch per width (device-resident rows)
+    t_per = 0.0
+    v_per = np.zeros(total, dtype=np.uint8)
+    dev = []
+    for f, (_, rows, bits_hi, _) in enumerate(parts):
+        d_n = hipmem.DeviceArray.from_host(rows)
+        d_o = hipmem.DeviceArray(len(rows), np.uint8)
+        engine.prp_device(d_n.ptr, rows.shape[1], len(rows), d_o.ptr, pgap.FERMAT2, (), bits_hi)
+        engine.sync()
+        engine.prp_device(d_n.ptr, rows.shape[1], len(rows), d_o.ptr, pgap.FERMAT2, (), bits_hi)
+        engine.sync()
+        t_per += engine.last_launch().kernel_ms
+        vv = d_o.to_host()
+        sel = np.nonzero(fam_of == f)[0]
+        v_per[sel] = vv[idx_in[sel]]
+        dev.append((d_n, d_o))
+    # one container: everything at the widest width
+    wmax = max(r.shape[1] for _, r, _, _ in parts)
+    allrows = np.zeros((total, wmax), dtype=np.uint32)
+    for f, (_, rows, _, _) in enumerate(parts):
+        sel = np.nonzero(fam_of == f)[0]
+        allrows[sel, : rows.shape[1]] = rows[idx_in[sel]]
+    d_all = hipmem.DeviceArray.from_host(allrows)
+    d_out = hipmem.DeviceArray(total, np.uint8)
+    bits_all = max(b for _, _, b, _ in parts)
+    engine.prp_device(d_all.ptr, wmax, total, d_out.ptr, pgap.FERMAT2, (), bits_all)
+    engine.sync()
+    engine.prp_device(d_all.ptr, wmax, total, d_out.ptr, pgap.FERMAT2, (), bits_all)
+    engine.sync()
+    t_one = engine.last_launch().kernel_ms
+    v_one = d_out.to_host()
+    return {
+        "workload": f"{total} candidates: 40% 503#/210 ({parts[0][3]:.0f} bits), 40% 1511#/2190 "
+                    f"({parts[1][3]:.0f} bits), 20% 809#/1841070 ({parts[2][3]:.0f} bits), m~U[1,1e6), "
+                    f"x~U[-20000,20000], no factor <= 1e6, seeded shuffle",
+        "persistent_tests_per_s": total / (t_mixed * 1e-3),
+        "per_width_tests_per_s": total / (t_per * 1e-3),
+        "one_container_tests_per_s": total / (t_one * 1e-3),
+        "kernel_ms": {"persistent": t_mixed, "per_width_sum": t_per, "one_container": t_one},
+        "verdicts_identical": bool(np.array_equal(v_mixed, v_per) and np.array_equal(v_mixed, v_one)),
+    }
+
+
+def bench_batch_4300(engine, hipmem, pgap, count, seed):
+    """Large operands: N = m*3001#/2190 + x, m ~ U[1, 1e5), x ~ U[-40000,
+    40000], no factor <= 1e6 (4231-4248 bits, quad-per-number variants)."""
+    rng = np.random.default_rng(seed)
+    rows, bits_hi, e_mean = sieved_family_rows(engine, hipmem, 3001, 2190, 1, 100_000, 40000, count, rng)
+    d_n = hipmem.DeviceArray.from_host(rows)
+    d_o = hipmem.DeviceArray(count, np.uint8)
+    ks = []
+    for i in range(3):
+        engine.prp_device(d_n.ptr, rows.shape[1], count, d_o.ptr, pgap.FERMAT2, (), bits_hi)
+        engine.sync()
+        if i:
+            ks.append(engine.last_launch().kernel_ms)
+    info = engine.last_launch()
+    avg = float(np.mean(ks)) * 1e-3
+    L32 = (int(e_mean + 0.999) + 31) // 32
+    return {
+        "workload": f"N = m*3001#/2190 + x, m~U[1,1e5), x~U[-40000,40000], no factor <= 1e6, {count} "
+                    f"candidates, {e_mean:.0f} bits",
+        "tests_per_s": count / avg,
+        "kernel_ms_avg": avg * 1e3,
+        "kernel": f"radix-2^{info.radix_bits} x {info.digits} digits, {info.lanes_per_number} lanes/number",
+        "roofline_frac": 2 * L32 * L32 * e_mean * count / avg / PEAK_INT32_MAC,
+    }
+
+
 def bench_stream(engine, pgap, n_m, load, seed, cpu_sample, threads):
     """Serving: P=1511, d=2190, SL=40000, m's in seeded random order from
     [1, 200000) coprime to d, each with its unknowns sieved to 1e9 (device
@@ -209,7 +332,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-2180", action="store_true", help="skip the 2180-bit batch")
     ap.add_argument("--no-stream", action="store_true", help="skip the serving measurement")
-    ap.add_argument("--stream-m", type=int, default=8192)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-width batch")
+    ap.add_argument("--no-4300", action="store_true", help="skip the 4300-bit batch")
+    ap.add_argument("--stream-m", type=int, default=16384)
     ap.add_argument("--stream-load", type=float, default=0.5)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (scales rocprofv3 FETCH/WRITE_SIZE)")
@@ -331,6 +456,10 @@ def main():
     extra = {}
     if not args.no_2180:
         extra["batch_2180_bit"] = bench_batch_2180(engine, hipmem, pgap, 1 << 18, 3, args.seed + 1)
+    if not args.no_mixed:
+        extra["mixed_widths"] = bench_mixed(engine, hipmem, pgap, 1 << 19, args.seed + 3)
+    if not args.no_4300:
+        extra["batch_4300_bit"] = bench_batch_4300(engine, hipmem, pgap, 1 << 15, args.seed + 4)
     if not args.no_stream:
         threads = min(16, os.cpu_count() or 1)
         extra["stream"] = bench_stream(engine, pgap, args.stream_m, args.stream_load, args.seed + 2,

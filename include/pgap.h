@@ -53,7 +53,8 @@ typedef struct {
     const char *tuning_path;  /* autotuner table written by pgap_autotune(); NULL = built-in defaults */
     int radix;                /* force 28 or 32 (0 = tuned) */
     int lanes_per_number;     /* force lanes per number (0 = tuned) */
-    int waves_per_simd;       /* force an occupancy target (0 = tuned) */
+    int waves_per_simd;       /* force an occupancy cap: resident 256-thread workgroups per CU, i.e.
+                                 waves per SIMD (0 = tuned, else the kernel's natural occupancy) */
 } pgap_options;
 
 /* Engine lifetime: owns the stream, device buffers and streaming state.
@@ -147,6 +148,7 @@ typedef struct {
 enum {
     PGAP_ROW_PREV_FALLBACK = 1,  /* no PRP among prev unknowns: searched beyond SL */
     PGAP_ROW_NEXT_FALLBACK = 2,
+    PGAP_ROW_NEXT_SKIPPED = 4,  /* side-skip: next side not scanned, next_p = n_tests = 0 */
 };
 
 /* One finished m: the row test_interval_cpu produces (gap_test_common.cpp:
@@ -175,6 +177,26 @@ size_t pgap_scan_in_flight(const pgap_engine *e);
  * speculative ones past a side's first PRP), rounds run, and the number of
  * slots the HBM budget allows. */
 pgap_status pgap_scan_stats(const pgap_engine *e, uint64_t *tests_issued, uint64_t *rounds, uint32_t *slots);
+
+/* Autotuner.  For each width in bits[0..n), times every kernel family that
+ * covers it (one lane per number, radix 2^28; a quad of lanes per number,
+ * radix 2^26) at its smallest covering size and under a few occupancy caps,
+ * for both tests, and keeps the fastest.  The table replaces the engine's
+ * current one (later batches and scans use it) and, when path is non-NULL,
+ * is written there for pgap_options.tuning_path.  Text format, one line per
+ * (test, width): "<test> <max_bits> <radix> <digits> <lanes> <cap> <tests/s>".
+ * Replaces the reference's compile-time choice of BITS / WINDOW_BITS /
+ * THREADS_PER_INSTANCE (gap_test_gpu.cu:48-81). */
+pgap_status pgap_autotune(pgap_engine *e, const uint32_t *bits, size_t n, const char *path);
+
+/* Side-skip, gap_test_gpu's order (gap_test_gpu.cu:591-617): each m scans
+ * its prev side first and starts the next side only after the prev PRP; when
+ * that PRP's merit, (float)(prev_p / (K_log + ln m)), is below
+ * min_merit / 2 - 2 the m finishes without the next side (row flag
+ * PGAP_ROW_NEXT_SKIPPED, next_p = n_tests = 0).  Off by default (both sides
+ * scan together, test_interval_cpu's result for every m).  Must be called
+ * before the first submit. */
+pgap_status pgap_scan_side_skip(pgap_engine *e, int enable, float min_merit);
 
 /* Streaming knobs: speculative depth per side (tests in flight per side),
  * Miller-Rabin confirmation rounds on each side's first PRP (bases 3, 5, ...),

@@ -106,14 +106,24 @@ pgap_status pgap_engine::ensure(DevBuf &b, size_t bytes, const char *what) {
 }
 
 pgap_status pgap_engine::pick(int kind, uint32_t bits, const Variant **out) {
-    const Variant *v = find_variant(kind, (int)bits, opt.radix, opt.lanes_per_number);
+    const Variant *v = nullptr;
+    const TuneEntry *t = (opt.radix || opt.lanes_per_number) ? nullptr : tuned(kind, bits);
+    if (t) v = find_variant(kind, (int)bits, t->radix, t->lanes);
+    if (!v) v = find_variant(kind, (int)bits, opt.radix, opt.lanes_per_number);
     if (!v && (opt.radix || opt.lanes_per_number)) v = find_variant(kind, (int)bits, 0, 0);
     if (!v) {
         set_error("no kernel variant for %u-bit numbers (kind %d)", bits, kind);
         return PGAP_E_TOO_LARGE;
     }
     *out = v;
+    cur_cap = cap_for(v, bits);
     return PGAP_OK;
+}
+
+int pgap_engine::cap_for(const Variant *v, uint32_t bits) const {
+    if (opt.waves_per_simd > 0) return opt.waves_per_simd;
+    const TuneEntry *t = tuned(v->kind, bits);
+    return (t && t->radix == v->radix && t->lanes == v->lanes && t->digits == v->digits) ? t->blocks_cap : 0;
 }
 
 pgap_status pgap_engine::launch_batch(const Variant *v, const uint32_t *d_n, uint32_t limbs, size_t count,
@@ -122,10 +132,11 @@ pgap_status pgap_engine::launch_batch(const Variant *v, const uint32_t *d_n, uin
     if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
     // Grid dimensions are 32-bit: split very large batches.
     const size_t chunk = (size_t)1 << 30;
+    const int cap = cur_cap;
     uint32_t launches = 0;
     for (size_t off = 0; off < count; off += chunk) {
         const uint32_t c = (uint32_t)std::min(chunk, count - off);
-        e = v->launch(d_n + off * limbs, limbs, c, nullptr, nullptr, bases, d_out + off, opt.waves_per_simd, stream);
+        e = v->launch(d_n + off * limbs, limbs, c, nullptr, nullptr, bases, d_out + off, cap, stream);
         if (e != hipSuccess) return hip_fail(e, "kernel launch");
         ++launches;
     }
@@ -226,6 +237,13 @@ pgap_status pgap_create(const pgap_options *opt, pgap_engine **out) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->opt.device) == hipSuccess && cus > 0)
         e->num_cus = cus;
+    if (e->opt.tuning_path && !e->load_tuning(e->opt.tuning_path)) {
+        e->set_error("cannot read tuning table %s", e->opt.tuning_path);
+        e->opt.tuning_path = nullptr;
+        *out = e;
+        return PGAP_E_ARG;
+    }
+    e->opt.tuning_path = nullptr;  // the caller's string need not outlive create
     e->ready = true;
     *out = e;
     return PGAP_OK;

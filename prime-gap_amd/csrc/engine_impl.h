@@ -20,6 +20,15 @@ struct DevBuf {
 };
 class Scanner;
 
+// One autotuner decision: numbers of up to `bits` bits of test `kind` run on
+// the variant (radix, digits, lanes) with an occupancy cap.
+struct TuneEntry {
+    int kind = 0;
+    uint32_t bits = 0;
+    int radix = 0, digits = 0, lanes = 0, blocks_cap = 0;
+    double rate = 0;
+};
+
 // K = P#/d with what the device needs to build and screen m*K + x.
 struct Family {
     uint32_t P = 0, D = 0, SL = 0;
@@ -43,6 +52,10 @@ struct pgap_engine {
     pgap_options opt{};
     bool ready = false;
     int num_cus = 256;  // compute units of the device (queried at create)
+    std::vector<pgap::TuneEntry> tune;  // autotuner table (sorted by kind, bits)
+    int cur_cap = 0;                    // occupancy cap of the variant last picked
+    int cap_for(const pgap::Variant *v, uint32_t bits) const;    bool load_tuning(const char *path);
+    const pgap::TuneEntry *tuned(int kind, uint32_t bits) const;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timing_pending = false;
@@ -71,6 +84,7 @@ struct pgap_engine {
     pgap_status hip_fail(hipError_t e, const char *what);
     pgap_status ensure(pgap::DevBuf &b, size_t bytes, const char *what);
     pgap_status pick(int kind, uint32_t bits, const pgap::Variant **out);
+    // launches on the variant last returned by pick(), with its occupancy cap
     pgap_status launch_batch(const pgap::Variant *v, const uint32_t *d_n, uint32_t limbs, size_t count,
                              const pgap::Bases &bases, uint8_t *d_out);
     void collect_timing();

@@ -189,7 +189,7 @@ pgap_status pgap_family(pgap_engine *e, uint32_t P, uint32_t D, uint32_t sieve_l
     f.D = D;
     f.SL = sieve_length;
     if (!primorial_over(P, D, f.K)) {
-        e->set_error("pgap_family: d=%u does not divide %u# (gap_common.cpp:119)", D, P);
+        e->set_error("pgap_family: d=%u must divide %u# with P#/d > 1 (gap_common.cpp:117-120)", D, P);
         return PGAP_E_ARG;
     }
     f.K_log = ln_words(f.K);

@@ -81,7 +81,7 @@ int main(int argc, char **argv) {
     if (!from_name) u.path = gen_unknown_fn(u);
     Words K;
     if (!primorial_over(u.p, u.d, K)) {
-        fprintf(stderr, "d=%u does not divide %u#\n", u.d, u.p);
+        fprintf(stderr, "d=%u must divide %u# with P#/d > 1\n", u.d, u.p);
         return 1;
     }
     pgap_engine *e = nullptr;

@@ -14,6 +14,10 @@
 // and rows are printed in m order as they complete; malformed input is an
 // error message and exit status 1, not an assert.
 //
+// --side-skip selects gap_test_gpu's order instead (gap_test_gpu.cu:591-617):
+// prev side first, and an m whose prev gap has merit below min_merit/2 - 2
+// ends without its next side (counted as "only prev_prime", no row).
+//
 // Extra flags: --depth N, --confirm N (scan knobs), --hbm-budget-mb N,
 // --results FILE (tab-separated P D m next_p prev_p merit, the columns of the
 // reference's result table, schema.sql:142-154), --dump-unknowns (parse and
@@ -47,6 +51,7 @@ struct Args {
     size_t hbm_budget = 0;
     std::string results;
     bool dump = false;
+    bool side_skip = false;
     bool valid = true;
 };
 
@@ -59,6 +64,7 @@ void usage(const char *name) {
     printf("  --depth <n>       speculative tests per side per round (default 8)\n");
     printf("  --confirm <n>     strong-PRP bases confirming each side's first PRP (default 2)\n");
     printf("  --hbm-budget-mb <n>  device memory the scan may keep resident\n");
+    printf("  --side-skip       gap_test_gpu order: prev side first, skip next if prev merit < min/2-2\n");
     printf("  --results <file>  append result rows (P D m next_p prev_p merit, tab separated)\n");
     printf("  --dump-unknowns   print the parsed unknown lists and exit (no GPU)\n");
     printf("  -q, --quiet       less output (twice for less)\n  -h, --help\n");
@@ -80,6 +86,7 @@ Args parse_args(int argc, char **argv) {
         {"max-prime", required_argument, 0, 5},          {"depth", required_argument, 0, 20},
         {"confirm", required_argument, 0, 21},           {"hbm-budget-mb", required_argument, 0, 22},
         {"results", required_argument, 0, 23},           {"dump-unknowns", no_argument, 0, 24},
+        {"side-skip", no_argument, 0, 25},
         {"method1", no_argument, 0, 8},                  {"quiet", no_argument, 0, 'q'},
         {"help", no_argument, 0, 'h'},                   {0, 0, 0, 0}};
     Args a;
@@ -103,6 +110,7 @@ Args parse_args(int argc, char **argv) {
             case 22: a.hbm_budget = (size_t)strtoull(optarg, nullptr, 10) << 20; break;
             case 23: a.results = optarg; break;
             case 24: a.dump = true; break;
+            case 25: a.side_skip = true; break;
             case 'u':
                 if (!parse_unknown_filename(optarg, a.u, err)) {
                     fprintf(stderr, "%s\n", err.c_str());
@@ -134,7 +142,7 @@ struct Stats {
     std::chrono::steady_clock::time_point start = std::chrono::steady_clock::now();
     uint32_t tests = 0;
     size_t total_unknown = 0, unk_prev = 0, unk_next = 0, prp_tests = 0;
-    size_t fallback_prev = 0, fallback_next = 0;
+    size_t fallback_prev = 0, fallback_next = 0, skips = 0;
     float best_merit = 0;
     long best_m = 0;
     double tests_per_second = 0;
@@ -169,6 +177,8 @@ struct Stats {
         printf("\t    prp tests %-10ld (avg: %.2f) (%.1f tests/sec)\n", (long)prp_tests, prp_tests / (float)tests,
                prp_tests / secs);
         if (verbose >= 2) {
+            if (skips)
+                printf("\t    only prev_prime %ld (%.2f%%)\n", (long)skips, 100.0 * skips / tests);
             if (fallback_prev + fallback_next > 0)
                 printf("\t    fallback prev_gap %ld (%.2f%%), next_gap %ld (%.2f%%)\n", (long)fallback_prev,
                        100.0 * fallback_prev / tests, (long)fallback_next, 100.0 * fallback_next / tests);
@@ -193,7 +203,7 @@ int main(int argc, char **argv) {
     }
     Words K;
     if (!primorial_over(a.u.p, a.u.d, K)) {
-        fprintf(stderr, "d=%u does not divide %u#\n", a.u.d, a.u.p);
+        fprintf(stderr, "d=%u must divide %u# with P#/d > 1\n", a.u.d, a.u.p);
         return 1;
     }
     const double K_log = ln_words(K);
@@ -265,6 +275,7 @@ int main(int argc, char **argv) {
             return 1;
         }
         if (pgap_scan_config(e, a.depth, a.confirm) != PGAP_OK ||
+            (a.side_skip && pgap_scan_side_skip(e, 1, a.min_merit) != PGAP_OK) ||
             pgap_family(e, a.u.p, a.u.d, a.u.sieve_length, &fam) != PGAP_OK) {
             fprintf(stderr, "scan set-up: %s\n", pgap_last_error(e));
             pgap_destroy(e);
@@ -293,10 +304,15 @@ int main(int argc, char **argv) {
             const long m = (long)r.m;
             const int prev_p = r.prev_p, next_p = r.next_p, gap = prev_p + next_p;
             const float merit = (float)(gap / (K_log + log((double)m)));
-            if (merit > min_merit)
-                printf("%-5d %.4f  %ld * %u#/%u -%d to +%d\n", gap, merit, m, a.u.p, a.u.d, prev_p, next_p);
-            if (results)
-                fprintf(results, "%u\t%u\t%ld\t%d\t%d\t%.4f\n", a.u.p, a.u.d, m, next_p, prev_p, merit);
+            const bool skipped = (r.flags & PGAP_ROW_NEXT_SKIPPED) != 0;
+            if (skipped) {
+                stats.skips += 1;  // the prev merit stands in (gap_test_gpu.cu:607-613)
+            } else {
+                if (merit > min_merit)
+                    printf("%-5d %.4f  %ld * %u#/%u -%d to +%d\n", gap, merit, m, a.u.p, a.u.d, prev_p, next_p);
+                if (results)
+                    fprintf(results, "%u\t%u\t%ld\t%d\t%d\t%.4f\n", a.u.p, a.u.d, m, next_p, prev_p, merit);
+            }
             stats.fallback_prev += (r.flags & PGAP_ROW_PREV_FALLBACK) ? 1 : 0;
             stats.fallback_next += (r.flags & PGAP_ROW_NEXT_FALLBACK) ? 1 : 0;
             const bool is_last = (r.m - a.u.mstart) == last_mi;

@@ -103,7 +103,8 @@ inline std::vector<uint32_t> primes_upto(uint32_t n) {
     return ps;
 }
 
-// K = P# / d; false if d does not divide P#.
+// K = P# / d; false if d does not divide P# or K <= 1 (init_K asserts both,
+// gap_common.cpp:117-120).
 inline bool primorial_over(uint32_t P, uint32_t D, Words &K) {
     K.assign(1, 1);
     for (uint32_t p : primes_upto(P)) mul_small(K, p);

@@ -24,9 +24,35 @@ enum Kind { KIND_FERMAT2 = 0, KIND_SPRP = 1 };
 //   d_count  if non-null, the actual count is read from device memory (so a
 //            round can size its work on the device without a host round trip)
 //   d_idx    if non-null, work item i tests row d_idx[i] and writes out[d_idx[i]]
+//   blocks_cap  if > 0, at most this many 256-thread workgroups per CU (an
+//            occupancy cap the autotuner can choose; enforced by reserving
+//            LDS the kernel does not use)
 typedef hipError_t (*batch_launch_fn)(const uint32_t *d_n, uint32_t limbs, uint32_t count, const uint32_t *d_count,
-                                      const uint32_t *d_idx, Bases bases, uint8_t *d_out, int waves_per_simd,
+                                      const uint32_t *d_idx, Bases bases, uint8_t *d_out, int blocks_cap,
                                       hipStream_t stream);
+
+// Dynamic LDS bytes that leave room for at most `cap` workgroups per CU
+// (160 KiB of LDS per CU on gfx950); 0 when cap <= 0.  Sets the kernel's
+// dynamic-LDS limit when the reservation exceeds the 64 KiB default.
+template <class K>
+inline size_t occupancy_cap_bytes(K kernel, int cap) {
+    if (cap <= 0) return 0;
+    const size_t lds_per_cu = 160 * 1024;
+    const size_t bytes = (lds_per_cu / (size_t)cap) & ~(size_t)255;  // cap + 1 groups would not fit
+    if (bytes > 64 * 1024)
+        (void)hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    return bytes;
+}
+
+// Streaming-scan launch: one grid runs a round's confirmation tests and its
+// Fermat tests together, so a round costs one test's latency.
+//   rows [0, nc)                       strong PRP to base d_base[i], nc = min(d_ctr[3], conf_max)
+//   rows [conf_rows, conf_rows + nf)   Fermat base 2, nf = min(d_ctr[0], max_items)
+// Confirmation items come first in the grid, padded to whole wavefronts, so
+// every wavefront runs one of the two tests.
+typedef hipError_t (*scan_launch_fn)(const uint32_t *d_n, uint32_t limbs, uint32_t conf_rows, uint32_t conf_max,
+                                     uint32_t max_items, const uint32_t *d_ctr, const uint8_t *d_base,
+                                     uint8_t *d_out, int blocks_cap, hipStream_t stream);
 
 struct Variant {
     int radix;       // 28 or 32
@@ -36,6 +62,7 @@ struct Variant {
     int max_bits;    // largest N the variant accepts (lazy reduction needs N < R/16)
     batch_launch_fn launch;
     int (*blocks_per_cu)();  // resident 256-thread workgroups per CU (occupancy query)
+    scan_launch_fn scan = nullptr;  // set on strong-PRP variants (max_bits is the SPRP limit)
 };
 
 // Registry: filled by every instantiation unit at library load.

@@ -118,6 +118,11 @@ __device__ __forceinline__ bool equal(const uint32_t (&a)[L], const uint32_t (&b
 // a <- a^2 * R^{-1} * 2^s mod N (s in {0,1}), lazy: a < 4N in, a < 4N out.
 // The 2^s factor is folded into the high (result) columns, so the doubling
 // step of a base-2 exponentiation costs two VALU slots per digit of output.
+//
+// Each column is summed in three independent chains -- the cross products,
+// and the m*N products in two halves -- issued round-robin and merged once
+// per column, so consecutive multiply-adds do not wait on each other (exact
+// integer sums: the result does not depend on the grouping).
 template <int L>
 __device__ __forceinline__ void sqr_redc(uint32_t (&a)[L], const uint32_t (&n)[L], uint32_t np,
                                          uint32_t s) {
@@ -127,24 +132,29 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&a)[L], const uint32_t (&n)[L
 #pragma unroll
     for (int k = 0; k < 2 * L - 1; ++k) {
         const int ilo = k - (L - 1) > 0 ? k - (L - 1) : 0;
-        // cross products a_i * a_{k-i}, i < k-i
-        uint64_t X = 0;
+        const int xe = (k + 1) / 2;        // cross products a_i a_{k-i}, i in [ilo, xe)
+        const int ye = k < L ? k : L;      // m_i n_{k-i}, i in [ilo, ye) (m_k joins below)
+        const int ym = (ilo + ye + 1) / 2; // split point of the m*N chain
+        const int nx = xe - ilo, n0 = ym - ilo, n1 = ye - ym;
+        const int rounds = nx > n0 ? (nx > n1 ? nx : n1) : (n0 > n1 ? n0 : n1);
+        uint64_t X = 0, Y0 = k < L ? T : 0, Y1 = 0;
 #pragma unroll
-        for (int i = ilo; 2 * i < k; ++i) X = mad(a[i], a[k - i], X);
+        for (int r = 0; r < rounds; ++r) {
+            if (r < nx) X = mad(a[ilo + r], a[k - ilo - r], X);
+            if (r < n0) Y0 = mad(m[ilo + r], n[k - ilo - r], Y0);
+            if (r < n1) Y1 = mad(m[ym + r], n[k - ym - r], Y1);
+        }
+        // opaque to the optimiser: otherwise reassociation folds the three
+        // sums back into one dependent chain
+        asm("" : "+v"(X), "+v"(Y0), "+v"(Y1));
+        uint64_t Z = (X << 1) + Y0;
+        Z += Y1;
+        if ((k & 1) == 0) Z = mad(a[k >> 1], a[k >> 1], Z);
         if (k < L) {
-            T = (X << 1) + T;
-            if ((k & 1) == 0) T = mad(a[k >> 1], a[k >> 1], T);
-#pragma unroll
-            for (int i = ilo; i < k; ++i) T = mad(m[i], n[k - i], T);
-            m[k] = ((uint32_t)T * np) & MASK;
-            T = mad(m[k], n[0], T);
+            m[k] = ((uint32_t)Z * np) & MASK;
+            T = mad(m[k], n[0], Z);
             T >>= 28;
         } else {
-            uint64_t Z = 0;
-            if ((k & 1) == 0) Z = mad(a[k >> 1], a[k >> 1], Z);
-#pragma unroll
-            for (int i = ilo; i < L; ++i) Z = mad(m[i], n[k - i], Z);
-            Z = (X << 1) + Z;
             if (k == L) T <<= s;
             T = (Z << s) + T;
             // digit k-L of the result; a[k-L] and m[k-L] are dead here
@@ -158,6 +168,12 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&a)[L], const uint32_t (&n)[L
 // Wide variant (L up to 127): the 2^s factor is applied to the finished
 // digits (a one-bit shift with carry) instead of to the column sums, whose
 // extra bit would break the 64-bit column bound above 63 digits.
+//
+// The wide variants run one wavefront per SIMD (their registers leave room for
+// no second one), so nothing hides the latency of a dependent multiply-add.
+// Each column is therefore summed in three independent chains -- the cross
+// products, and the m*N products in two halves -- issued round-robin, and
+// merged once per column (exact integer sums: the result is unchanged).
 template <int L>
 __device__ __forceinline__ void sqr_redc_wide(uint32_t (&a)[L], const uint32_t (&n)[L], uint32_t np,
                                               uint32_t s) {
@@ -167,20 +183,29 @@ __device__ __forceinline__ void sqr_redc_wide(uint32_t (&a)[L], const uint32_t (
 #pragma unroll
     for (int k = 0; k < 2 * L - 1; ++k) {
         const int ilo = k - (L - 1) > 0 ? k - (L - 1) : 0;
-        uint64_t X = 0;
+        const int xe = (k + 1) / 2;        // cross products a_i a_{k-i}, i in [ilo, xe)
+        const int ye = k < L ? k : L;      // m_i n_{k-i}, i in [ilo, ye) (m_k joins below)
+        const int ym = (ilo + ye + 1) / 2; // split point of the m*N chain
+        const int nx = xe - ilo, n0 = ym - ilo, n1 = ye - ym;
+        const int rounds = nx > n0 ? (nx > n1 ? nx : n1) : (n0 > n1 ? n0 : n1);
+        uint64_t X = 0, Y0 = T, Y1 = 0;
 #pragma unroll
-        for (int i = ilo; 2 * i < k; ++i) X = mad(a[i], a[k - i], X);
-        T = (X << 1) + T;
+        for (int r = 0; r < rounds; ++r) {
+            if (r < nx) X = mad(a[ilo + r], a[k - ilo - r], X);
+            if (r < n0) Y0 = mad(m[ilo + r], n[k - ilo - r], Y0);
+            if (r < n1) Y1 = mad(m[ym + r], n[k - ym - r], Y1);
+        }
+        // opaque to the optimiser: otherwise reassociation folds the three
+        // sums back into one dependent chain
+        asm("" : "+v"(X), "+v"(Y0), "+v"(Y1));
+        T = (X << 1) + Y0;
+        T += Y1;
         if ((k & 1) == 0) T = mad(a[k >> 1], a[k >> 1], T);
         if (k < L) {
-#pragma unroll
-            for (int i = ilo; i < k; ++i) T = mad(m[i], n[k - i], T);
             m[k] = ((uint32_t)T * np) & MASK;
             T = mad(m[k], n[0], T);
             T >>= 28;
         } else {
-#pragma unroll
-            for (int i = ilo; i < L; ++i) T = mad(m[i], n[k - i], T);
             a[k - L] = (uint32_t)T & MASK;
             T >>= 28;
         }

@@ -23,20 +23,25 @@ __device__ __forceinline__ void load_n(const uint32_t* __restrict__ nw, uint32_t
     from_words<L, W>(w, n);
 }
 
-// Shared set-up: Mont(1) = R mod N and y = Mont(2) = 2R mod N, built by
-// modular doublings from 2^(b-1) < N (no division needed).
+// Mont(1) = R mod N for a b-bit N, built by modular doublings from
+// 2^(b-1) < N (no division needed): 28L - (b-1) doublings.
 template <int L>
-__device__ __forceinline__ void mont_one_two(const uint32_t (&n)[L], int b, uint32_t (&one)[L],
-                                             uint32_t (&two)[L]) {
+__device__ __forceinline__ void mont_one(const uint32_t (&n)[L], int b, uint32_t (&one)[L]) {
     const int top = b - 1;
     const int td = top / 28, tb = top - 28 * td;
 #pragma unroll
-    for (int j = 0; j < L; ++j) two[j] = (j == td) ? (1u << tb) : 0u;
-    // 2^(b-1) -> 2^(28L) mod N takes 28L - (b-1) doublings.
+    for (int j = 0; j < L; ++j) one[j] = (j == td) ? (1u << tb) : 0u;
     const int cnt = 28 * L - top;
-    for (int c = 0; c < cnt; ++c) mod_double<L>(two, n);
+    for (int c = 0; c < cnt; ++c) mod_double<L>(one, n);
+}
+
+// Shared set-up: Mont(1) = R mod N and Mont(2) = 2R mod N.
+template <int L>
+__device__ __forceinline__ void mont_one_two(const uint32_t (&n)[L], int b, uint32_t (&one)[L],
+                                             uint32_t (&two)[L]) {
+    mont_one<L>(n, b, one);
 #pragma unroll
-    for (int j = 0; j < L; ++j) one[j] = two[j];
+    for (int j = 0; j < L; ++j) two[j] = one[j];
     mod_double<L>(two, n);
 }
 
@@ -75,8 +80,11 @@ __device__ __forceinline__ uint32_t fermat2(const uint32_t* __restrict__ nw, uin
     const int b = bitlen<L>(n);
     if (b < 2) return 0;
     const uint32_t np = neg_inv28(n[0]);
-    uint32_t one[L], x[L];
-    mont_one_two<L>(n, b, one, x);
+    // x = Mont(2); Mont(1) is rebuilt after the loop rather than kept, which
+    // frees L registers for the whole exponentiation
+    uint32_t x[L];
+    mont_one<L>(n, b, x);
+    mod_double<L>(x, n);
 
     int i = b - 2;
     uint32_t ew = nw[i >> 5];
@@ -89,6 +97,8 @@ __device__ __forceinline__ uint32_t fermat2(const uint32_t* __restrict__ nw, uin
         --i;
     }
     cond_sub<L>(x, n);
+    uint32_t one[L];
+    mont_one<L>(n, b, one);
     return equal<L>(x, one) ? 1u : 0u;
 }
 

@@ -31,12 +31,48 @@ __global__ void __launch_bounds__(256) prp_r28_kernel(const uint32_t *__restrict
 
 template <int L, int KIND>
 hipError_t launch_r28(const uint32_t *d_n, uint32_t limbs, uint32_t count, const uint32_t *d_count,
-                      const uint32_t *d_idx, Bases bases, uint8_t *d_out, int /*waves_per_simd*/,
-                      hipStream_t stream) {
+                      const uint32_t *d_idx, Bases bases, uint8_t *d_out, int blocks_cap, hipStream_t stream) {
     if (count == 0) return hipSuccess;
     const dim3 grid((count + 255u) / 256u);
-    hipLaunchKernelGGL((prp_r28_kernel<L, KIND>), grid, dim3(256), 0, stream, d_n, limbs, count, d_count, d_idx,
+    const size_t lds = occupancy_cap_bytes(prp_r28_kernel<L, KIND>, blocks_cap);
+    hipLaunchKernelGGL((prp_r28_kernel<L, KIND>), grid, dim3(256), lds, stream, d_n, limbs, count, d_count, d_idx,
                        bases, d_out);
+    return hipGetLastError();
+}
+
+// Streaming-scan round: confirmation items (one strong-PRP base each) in
+// whole wavefronts first, then Fermat items.
+template <int L>
+__global__ void __launch_bounds__(256) scan_r28_kernel(const uint32_t *__restrict__ n, uint32_t limbs,
+                                                       uint32_t conf_rows, uint32_t conf_max, uint32_t max_items,
+                                                       const uint32_t *__restrict__ ctr,
+                                                       const uint8_t *__restrict__ base, uint8_t *__restrict__ out) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t nc = min(ctr[3], conf_max), nf = min(ctr[0], max_items);
+    const uint32_t ncw = (nc + 63u) & ~63u;
+    uint32_t row, v;
+    if (t < ncw) {
+        if (t >= nc) return;
+        row = t;
+        v = r28::sprp<L>(n + (size_t)row * limbs, limbs, base[t]);
+    } else {
+        if (t - ncw >= nf) return;
+        row = conf_rows + (t - ncw);
+        v = r28::fermat2<L>(n + (size_t)row * limbs, limbs);
+    }
+    out[row] = (uint8_t)v;
+}
+
+template <int L>
+hipError_t launch_scan_r28(const uint32_t *d_n, uint32_t limbs, uint32_t conf_rows, uint32_t conf_max,
+                           uint32_t max_items, const uint32_t *d_ctr, const uint8_t *d_base, uint8_t *d_out,
+                           int blocks_cap, hipStream_t stream) {
+    const size_t threads = (((size_t)conf_max + 63) & ~(size_t)63) + max_items;
+    if (threads == 0) return hipSuccess;
+    const dim3 grid((uint32_t)((threads + 255) / 256));
+    const size_t lds = occupancy_cap_bytes(scan_r28_kernel<L>, blocks_cap);
+    hipLaunchKernelGGL((scan_r28_kernel<L>), grid, dim3(256), lds, stream, d_n, limbs, conf_rows, conf_max,
+                       max_items, d_ctr, d_base, d_out);
     return hipGetLastError();
 }
 
@@ -54,8 +90,8 @@ struct RegisterR28 {
         (register_variant(Variant{28, Ls, 1, KIND_FERMAT2, 28 * Ls - 4, &launch_r28<Ls, KIND_FERMAT2>,
                                   &occupancy_r28<Ls, KIND_FERMAT2>}),
          ...);
-        (register_variant(
-             Variant{28, Ls, 1, KIND_SPRP, 28 * Ls - 14, &launch_r28<Ls, KIND_SPRP>, &occupancy_r28<Ls, KIND_SPRP>}),
+        (register_variant(Variant{28, Ls, 1, KIND_SPRP, 28 * Ls - 14, &launch_r28<Ls, KIND_SPRP>,
+                                  &occupancy_r28<Ls, KIND_SPRP>, &launch_scan_r28<Ls>}),
          ...);
     }
 };

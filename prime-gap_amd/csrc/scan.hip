@@ -10,10 +10,16 @@
 // Design.  Every admitted m owns a slot: its centre m*K, its prev and next
 // unknown lists (outward order) and per-side state (next index to test,
 // index of the first confirmed PRP) stay in HBM until the m finishes.  A
-// round tests, for every unfinished side, the next `depth` unknowns at once:
-//   issue -> materialise N = centre -+ x -> Fermat base 2 -> collect hits ->
-//   strong-PRP confirmation of the hits -> first confirmed hit per side ->
-//   per-slot update, emitting finished m's and exhausted sides.
+// round tests, for every unfinished side, either the next `depth` unknowns
+// (Fermat base 2) or, when the side's first Fermat hit is awaiting
+// confirmation, that hit against the next strong-PRP base:
+//   issue -> materialise N = centre -+ x -> one grid running the round's
+//   confirmations and Fermat tests -> retire (first Fermat hit per side,
+//   confirmation stage) -> per-slot update, emitting finished m's and
+//   exhausted sides.
+// Confirmations ride in the next round's grid instead of a launch of their
+// own, so a round costs one test's latency.  A hit that fails a
+// confirmation (a base-2 pseudoprime) restarts its side just past it.
 // Every index below a side's first PRP has been tested in this or an earlier
 // round, so "first confirmed PRP" is exactly the reference's first
 // mpz_probab_prime_p hit; tests issued past it in the same round are
@@ -25,6 +31,7 @@
 // that arrive while others are on the device join the next round.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -44,14 +51,20 @@ enum : uint32_t {
     F_NEXT_DONE = 2,
     F_PREV_WAIT = 4,  // exhausted, waiting for the host to append candidates
     F_NEXT_WAIT = 8,
+    F_SEQ = 16,        // side-skip mode: the next side starts after the prev side's PRP
+    F_NEXT_SKIP = 32,  // side-skip taken: the next side was not scanned
 };
 
 struct SlotState {
     uint32_t cursor[2];
-    uint32_t found[2];
+    uint32_t found[2];  // index of the side's confirmed PRP (NONE until then)
     uint32_t len[2];
     uint32_t flags;
     uint32_t cwords;
+    uint32_t cand[2];   // first Fermat hit awaiting confirmation (NONE if none)
+    uint32_t stage[2];  // confirmation bases it has passed
+    uint32_t skip_below;  // F_SEQ: a prev distance below this skips the next side
+    uint32_t pad;
 };
 
 // Event record written for a slot that finished or needs fallback candidates.
@@ -59,7 +72,8 @@ struct Event {
     uint32_t slot, code;  // code bit 0: finished; bit 1/2: prev/next exhausted
     uint32_t found[2];
     int32_t x[2];
-    uint32_t pad[2];
+    uint32_t flags;       // slot flags when the event was written
+    uint32_t pad;
 };
 
 // Admission / append descriptors (host -> device, one per request).
@@ -71,7 +85,9 @@ struct Upload {
     uint32_t n0, n1;      // list entries (kind 0: prev, next; kind 1: appended)
     uint32_t off_center;  // word offsets into the data part
     uint32_t off_l0, off_l1;
-    uint32_t pad[3];
+    uint32_t init_flags;  // kind 0: initial slot flags (F_SEQ)
+    uint32_t skip_below;  // kind 0: see SlotState
+    uint32_t pad;
 };
 
 // ---- kernels --------------------------------------------------------------
@@ -93,9 +109,13 @@ __global__ void scatter_kernel(const Upload *__restrict__ up, const uint32_t *__
             SlotState s;
             s.cursor[0] = s.cursor[1] = 0;
             s.found[0] = s.found[1] = NONE;
+            s.cand[0] = s.cand[1] = NONE;
+            s.stage[0] = s.stage[1] = 0;
             s.len[0] = d.n0;
             s.len[1] = d.n1;
-            s.flags = 0;
+            s.flags = d.init_flags;
+            s.skip_below = d.skip_below;
+            s.pad = 0;
             s.cwords = d.cwords;
             st[d.slot] = s;
         }
@@ -107,14 +127,19 @@ __global__ void scatter_kernel(const Upload *__restrict__ up, const uint32_t *__
         __syncthreads();
         if (threadIdx.x == 0) {
             st[d.slot].len[d.side] = base + d.n0;
-            st[d.slot].flags &= ~(d.side ? F_NEXT_WAIT : F_PREV_WAIT);
+            // both sides of one slot can be appended in the same launch (two
+            // blocks): clear this side's bit without losing the other's
+            atomicAnd(&st[d.slot].flags, ~(d.side ? F_NEXT_WAIT : F_PREV_WAIT));
         }
     }
 }
 
-// One thread per (active slot, side): reserve up to `depth` items.
+// One thread per (active slot, side): a confirmation item for a pending
+// Fermat hit (rows [0, 2 na), counter ctr[3]) or up to `depth` Fermat items
+// (rows 2 na + [0, ctr[0])).
 __global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, SlotState *__restrict__ st,
-                             uint32_t depth, uint32_t *__restrict__ items, uint32_t *__restrict__ item_idx,
+                             uint32_t depth, Bases conf, uint32_t *__restrict__ items,
+                             uint32_t *__restrict__ item_idx, uint8_t *__restrict__ item_base,
                              uint32_t *__restrict__ ctr) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= 2 * na) return;
@@ -123,10 +148,18 @@ __global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, S
     const uint32_t done = side ? F_NEXT_DONE : F_PREV_DONE;
     const uint32_t wait = side ? F_NEXT_WAIT : F_PREV_WAIT;
     if (s.flags & (done | wait)) return;
+    if (side == 1 && (s.flags & (F_SEQ | F_PREV_DONE)) == F_SEQ) return;  // next waits for prev
+    if (s.cand[side] != NONE) {
+        const uint32_t r = atomicAdd(&ctr[3], 1u);
+        items[r] = (slot << 1) | side;
+        item_idx[r] = s.cand[side];
+        item_base[r] = (uint8_t)conf.b[s.stage[side]];
+        return;
+    }
     const uint32_t c = s.cursor[side], l = s.len[side];
     if (c >= l) return;
     const uint32_t k = min(depth, l - c);
-    const uint32_t base = atomicAdd(&ctr[0], k);
+    const uint32_t base = 2 * na + atomicAdd(&ctr[0], k);
     for (uint32_t j = 0; j < k; ++j) {
         items[base + j] = (slot << 1) | side;
         item_idx[base + j] = c + j;
@@ -134,13 +167,19 @@ __global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, S
     s.cursor[side] = c + k;
 }
 
+// Rows in use this round: confirmations [0, ctr[3]), Fermat 2 na + [0, ctr[0]).
+__device__ __forceinline__ bool row_live(uint32_t t, uint32_t cr, const uint32_t *ctr) {
+    return t < cr ? t < ctr[3] : t - cr < ctr[0];
+}
+
 // N = centre -+ x for every item, `w` words per row.
 __global__ void materialize_kernel(const uint32_t *__restrict__ items, const uint32_t *__restrict__ item_idx,
-                                   const uint32_t *__restrict__ ctr, const uint32_t *__restrict__ center,
-                                   uint32_t cw, const int32_t *__restrict__ lists, uint32_t cap,
-                                   uint32_t *__restrict__ nbuf, uint32_t w) {
+                                   uint32_t cr, uint32_t nrows, const uint32_t *__restrict__ ctr,
+                                   const uint32_t *__restrict__ center, uint32_t cw,
+                                   const int32_t *__restrict__ lists, uint32_t cap, uint32_t *__restrict__ nbuf,
+                                   uint32_t w) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= ctr[0]) return;
+    if (t >= nrows || !row_live(t, cr, ctr)) return;
     const uint32_t slot = items[t] >> 1, side = items[t] & 1u;
     const uint32_t x = (uint32_t)lists[((size_t)slot * 2 + side) * cap + item_idx[t]];
     const uint32_t *c = center + (size_t)slot * cw;
@@ -162,20 +201,30 @@ __global__ void materialize_kernel(const uint32_t *__restrict__ items, const uin
     }
 }
 
-__global__ void hits_kernel(const uint8_t *__restrict__ verdict, uint32_t *__restrict__ ctr,
-                            uint32_t *__restrict__ hits) {
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= ctr[0]) return;
-    if (verdict[t]) hits[atomicAdd(&ctr[1], 1u)] = t;
-}
-
+// Confirmation rows advance or drop their side's pending hit; Fermat hits
+// become the side's pending hit (or, with no confirmation rounds, its PRP).
+// A side has either one confirmation row or Fermat rows in a round, never
+// both, so only the Fermat hits need atomics.
 __global__ void retire_kernel(const uint32_t *__restrict__ items, const uint32_t *__restrict__ item_idx,
-                              const uint8_t *__restrict__ verdict, const uint32_t *__restrict__ ctr,
+                              uint32_t cr, uint32_t nrows, const uint8_t *__restrict__ verdict,
+                              const uint32_t *__restrict__ ctr, uint32_t confirm_rounds,
                               SlotState *__restrict__ st) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= ctr[0] || !verdict[t]) return;
+    if (t >= nrows || !row_live(t, cr, ctr)) return;
     const uint32_t slot = items[t] >> 1, side = items[t] & 1u;
-    atomicMin(&st[slot].found[side], item_idx[t]);
+    SlotState &s = st[slot];
+    if (t < cr) {
+        if (verdict[t]) {
+            if (++s.stage[side] >= confirm_rounds) s.found[side] = s.cand[side];
+        } else {
+            s.cursor[side] = s.cand[side] + 1;
+            s.cand[side] = NONE;
+            s.stage[side] = 0;
+        }
+        return;
+    }
+    if (!verdict[t]) return;
+    atomicMin(confirm_rounds ? &s.cand[side] : &s.found[side], item_idx[t]);
 }
 
 __global__ void update_kernel(const uint32_t *__restrict__ active, uint32_t na, SlotState *__restrict__ st,
@@ -190,9 +239,14 @@ __global__ void update_kernel(const uint32_t *__restrict__ active, uint32_t na, 
         const uint32_t done = side ? F_NEXT_DONE : F_PREV_DONE;
         const uint32_t wait = side ? F_NEXT_WAIT : F_PREV_WAIT;
         if (flags & (done | wait)) continue;
+        if (side == 1 && (flags & (F_SEQ | F_PREV_DONE)) == F_SEQ) continue;  // not started
         if (s.found[side] != NONE) {
             flags |= done;
-        } else if (s.cursor[side] >= s.len[side]) {
+            // side-skip (gap_test_gpu.cu:591-617): a small prev gap ends the m
+            if (side == 0 && (flags & F_SEQ) &&
+                (uint32_t)lists[(size_t)slot * 2 * cap + s.found[0]] < s.skip_below)
+                flags |= F_NEXT_DONE | F_NEXT_SKIP;
+        } else if (s.cand[side] == NONE && s.cursor[side] >= s.len[side]) {
             flags |= wait;
             code |= 2u << side;
         }
@@ -207,7 +261,8 @@ __global__ void update_kernel(const uint32_t *__restrict__ active, uint32_t na, 
         ev.found[side] = s.found[side];
         ev.x[side] = s.found[side] != NONE ? lists[((size_t)slot * 2 + side) * cap + s.found[side]] : 0;
     }
-    ev.pad[0] = ev.pad[1] = 0;
+    ev.flags = flags;
+    ev.pad = 0;
     events[atomicAdd(&ctr[2], 1u)] = ev;
 }
 
@@ -240,6 +295,8 @@ class Scanner {
   public:
     pgap_engine *e;
     uint32_t depth = 32, confirm_rounds = 2;
+    bool side_skip = false;       // gap_test_gpu's prev-first order with side-skip
+    float skip_min_merit = 0;     // min_merit / 2 - 2 (float, gap_test_gpu.cu:604)
     uint32_t S = 0, CW = 0, CAP = 0, W = 0, MAXI = 0;
     bool inited = false;
 
@@ -257,9 +314,9 @@ class Scanner {
 
     SlotState *d_state = nullptr;
     uint32_t *d_center = nullptr, *d_active = nullptr, *d_items = nullptr, *d_idx = nullptr, *d_nbuf = nullptr,
-             *d_hits = nullptr, *d_ctr = nullptr, *d_updata = nullptr;
+             *d_ctr = nullptr, *d_updata = nullptr;
     int32_t *d_lists = nullptr;
-    uint8_t *d_verdict = nullptr;
+    uint8_t *d_verdict = nullptr, *d_base = nullptr;
     Event *d_events = nullptr;
     Upload *d_up = nullptr;
     size_t up_cap = 0, updata_cap = 0;
@@ -278,15 +335,15 @@ class Scanner {
     void release() {
         (void)hipStreamSynchronize(e->stream);
         for (void *p : {(void *)d_state, (void *)d_center, (void *)d_active, (void *)d_items, (void *)d_idx,
-                        (void *)d_nbuf, (void *)d_hits, (void *)d_ctr, (void *)d_lists, (void *)d_verdict,
+                        (void *)d_nbuf, (void *)d_base, (void *)d_ctr, (void *)d_lists, (void *)d_verdict,
                         (void *)d_events, (void *)d_up, (void *)d_updata})
             if (p) (void)hipFree(p);
         for (void *p : {(void *)h_active, (void *)h_ctr, (void *)h_events, (void *)h_up, (void *)h_updata})
             if (p) (void)hipHostFree(p);
         d_state = nullptr;
-        d_center = d_active = d_items = d_idx = d_nbuf = d_hits = d_ctr = d_updata = nullptr;
+        d_center = d_active = d_items = d_idx = d_nbuf = d_ctr = d_updata = nullptr;
         d_lists = nullptr;
-        d_verdict = nullptr;
+        d_verdict = d_base = nullptr;
         d_events = nullptr;
         d_up = nullptr;
         h_active = h_ctr = h_updata = nullptr;
@@ -336,8 +393,10 @@ class Scanner {
         CW = kw + 2;   // m < 2^64
         W = CW + 1;    // room for the carry of + x
         CAP = std::max<uint32_t>(sl, 1) + 4 * FALLBACK_CHUNK;
+        // per side: `depth` Fermat rows and one confirmation row (number,
+        // item, index, verdict; base for the confirmation row)
         const size_t per_slot = sizeof(SlotState) + 4ull * CW + 8ull * CAP + 8 /*active*/ + sizeof(Event) +
-                                2ull * depth_max() * (4ull * W + 4 + 4 + 1 + 4);
+                                2ull * (depth_max() + 1) * (4ull * W + 4 + 4 + 1) + 2;
         size_t budget = e->opt.hbm_budget;
         if (budget == 0) {
             size_t fr = 0, tot = 0;
@@ -350,12 +409,13 @@ class Scanner {
             return PGAP_E_TOO_LARGE;
         }
         MAXI = 2 * S * depth_max();
+        const size_t rows = (size_t)2 * S + MAXI;  // confirmation rows, then Fermat rows
         pgap_status st;
         if ((st = dalloc(&d_state, S, "state")) || (st = dalloc(&d_center, (size_t)S * CW, "centres")) ||
             (st = dalloc(&d_lists, (size_t)S * 2 * CAP, "lists")) || (st = dalloc(&d_active, S, "active")) ||
-            (st = dalloc(&d_items, MAXI, "items")) || (st = dalloc(&d_idx, MAXI, "item index")) ||
-            (st = dalloc(&d_nbuf, (size_t)MAXI * W, "numbers")) || (st = dalloc(&d_verdict, MAXI, "verdicts")) ||
-            (st = dalloc(&d_hits, MAXI, "hits")) || (st = dalloc(&d_ctr, 4, "counters")) ||
+            (st = dalloc(&d_items, rows, "items")) || (st = dalloc(&d_idx, rows, "item index")) ||
+            (st = dalloc(&d_nbuf, rows * W, "numbers")) || (st = dalloc(&d_verdict, rows, "verdicts")) ||
+            (st = dalloc(&d_base, (size_t)2 * S, "confirmation bases")) || (st = dalloc(&d_ctr, 4, "counters")) ||
             (st = dalloc(&d_events, S, "events")) || (st = halloc(&h_active, S, "active")) ||
             (st = halloc(&h_ctr, 4, "counters")) || (st = halloc(&h_events, S, "events"))) {
             release();
@@ -369,6 +429,19 @@ class Scanner {
         fb_kmod.clear();
         inited = true;
         return PGAP_OK;
+    }
+
+    // Smallest prev distance d whose merit, computed as gap_test_gpu.cu:595
+    // does (float)(d / (K_log + log m)), is not below skip_min_merit: a prev
+    // PRP nearer than this skips the next side.
+    uint32_t skip_distance(double K_log, uint64_t m) const {
+        const double den = K_log + std::log((double)m);
+        const auto below = [&](double d) { return (float)(d / den) < skip_min_merit; };
+        double d = std::floor((double)skip_min_merit * den) - 2;
+        if (d < 0) d = 0;
+        while (d > 0 && !below(d - 1)) d -= 1;
+        while (below(d)) d += 1;
+        return d > 4294967295.0 ? 0xffffffffu : (uint32_t)d;
     }
 
     const std::vector<uint32_t> &kmod_small(int family) {
@@ -472,6 +545,10 @@ class Scanner {
             d.slot = slot;
             d.kind = 0;
             d.cwords = (uint32_t)c.size();
+            if (side_skip) {
+                d.init_flags = F_SEQ;
+                d.skip_below = skip_distance(f.K_log, r.m);
+            }
             d.n0 = sh.len[0];
             d.n1 = sh.len[1];
             d.off_center = (uint32_t)off;
@@ -519,14 +596,20 @@ class Scanner {
         const Variant *vf, *vc = nullptr;
         if ((st = e->pick(KIND_FERMAT2, bits, &vf))) return st;
         if (confirm_rounds && (st = e->pick(KIND_SPRP, bits, &vc))) return st;
+        if (vc && !vc->scan) {
+            e->set_error("scan: variant radix %d x %d has no scan kernel", vc->radix, vc->digits);
+            return PGAP_E_STATE;
+        }
+        const int cap_f = e->cap_for(vf, bits), cap_c = vc ? e->cap_for(vc, bits) : 0;
         // Speculative depth.  A round costs about one test's latency whether
         // it carries one item or a full device's worth (the kernel is VALU
         // bound per lane), so every round is sized to the numbers the chosen
         // variant keeps resident at once: few active sides get a deep
         // speculative window, many get one or two tests each.
+        const Variant *vr = vc ? vc : vf;
         const uint32_t resident =
-            (uint32_t)std::max(1, vf->blocks_per_cu ? vf->blocks_per_cu() : 1) * e->num_cus * 256u /
-            (uint32_t)std::max(1, vf->lanes);
+            (uint32_t)std::max(1, vr->blocks_per_cu ? vr->blocks_per_cu() : 1) * e->num_cus * 256u /
+            (uint32_t)std::max(1, vr->lanes);
         const uint32_t d =
             std::max<uint32_t>(1, std::min<uint32_t>(depth_max(), (resident + 2 * na - 1) / (2 * na)));
         Bases none{}, conf{};
@@ -539,24 +622,23 @@ class Scanner {
         if ((he = hipMemcpyAsync(d_active, h_active, na * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
             (he = hipMemsetAsync(d_ctr, 0, 16, e->stream)) != hipSuccess)
             return e->hip_fail(he, "scan round set-up");
-        const uint32_t maxi = 2 * na * d;
+        const uint32_t cr = 2 * na;     // confirmation rows (at most one per side)
+        const uint32_t maxi = 2 * na * d;  // Fermat rows
+        const uint32_t nrows = cr + maxi;
         const dim3 b256(256);
         hipLaunchKernelGGL(issue_kernel, dim3((2 * na + 255) / 256), b256, 0, e->stream, d_active, na, d_state, d,
-                           d_items, d_idx, d_ctr);
-        hipLaunchKernelGGL(materialize_kernel, dim3((maxi + 255) / 256), b256, 0, e->stream, d_items, d_idx, d_ctr,
-                           d_center, CW, d_lists, CAP, d_nbuf, W);
+                           conf, d_items, d_idx, d_base, d_ctr);
+        hipLaunchKernelGGL(materialize_kernel, dim3((nrows + 255) / 256), b256, 0, e->stream, d_items, d_idx, cr,
+                           nrows, d_ctr, d_center, CW, d_lists, CAP, d_nbuf, W);
         if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "scan issue");
-        if ((he = vf->launch(d_nbuf, W, maxi, d_ctr, nullptr, none, d_verdict, e->opt.waves_per_simd, e->stream)) !=
-            hipSuccess)
-            return e->hip_fail(he, "scan Fermat launch");
-        if (vc) {
-            hipLaunchKernelGGL(hits_kernel, dim3((maxi + 255) / 256), b256, 0, e->stream, d_verdict, d_ctr, d_hits);
-            if ((he = vc->launch(d_nbuf, W, maxi, d_ctr + 1, d_hits, conf, d_verdict, e->opt.waves_per_simd,
-                                 e->stream)) != hipSuccess)
-                return e->hip_fail(he, "scan confirm launch");
-        }
-        hipLaunchKernelGGL(retire_kernel, dim3((maxi + 255) / 256), b256, 0, e->stream, d_items, d_idx, d_verdict,
-                           d_ctr, d_state);
+        if (vc)
+            he = vc->scan(d_nbuf, W, cr, cr, maxi, d_ctr, d_base, d_verdict, cap_c, e->stream);
+        else
+            he = vf->launch(d_nbuf + (size_t)cr * W, W, maxi, d_ctr, nullptr, none, d_verdict + cr, cap_f,
+                            e->stream);
+        if (he != hipSuccess) return e->hip_fail(he, "scan test launch");
+        hipLaunchKernelGGL(retire_kernel, dim3((nrows + 255) / 256), b256, 0, e->stream, d_items, d_idx, cr, nrows,
+                           d_verdict, d_ctr, confirm_rounds, d_state);
         hipLaunchKernelGGL(update_kernel, dim3((na + 255) / 256), b256, 0, e->stream, d_active, na, d_state, d_lists,
                            CAP, d_ctr, d_events);
         if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "scan retire");
@@ -564,7 +646,7 @@ class Scanner {
             (he = hipStreamSynchronize(e->stream)) != hipSuccess)
             return e->hip_fail(he, "scan round");
         const uint32_t nev = h_ctr[2];
-        tests_issued += h_ctr[0];
+        tests_issued += (uint64_t)h_ctr[0] + h_ctr[3];
         ++rounds;
         if (nev) {
             if ((he = hipMemcpy(h_events, d_events, nev * sizeof(Event), hipMemcpyDeviceToHost)) != hipSuccess)
@@ -603,8 +685,9 @@ class Scanner {
                 r.prev_p = ev.x[0];
                 r.next_p = ev.x[1];
                 r.p_tests = std::min(ev.found[0] + 1, sh.orig_len[0]);
-                r.n_tests = std::min(ev.found[1] + 1, sh.orig_len[1]);
-                r.flags = (sh.fb[0] ? PGAP_ROW_PREV_FALLBACK : 0) | (sh.fb[1] ? PGAP_ROW_NEXT_FALLBACK : 0);
+                r.n_tests = ev.found[1] == NONE ? 0 : std::min(ev.found[1] + 1, sh.orig_len[1]);
+                r.flags = (sh.fb[0] ? PGAP_ROW_PREV_FALLBACK : 0) | (sh.fb[1] ? PGAP_ROW_NEXT_FALLBACK : 0) |
+                          ((ev.flags & F_NEXT_SKIP) ? PGAP_ROW_NEXT_SKIPPED : 0);
                 r.t_submit = sh.t_submit;
                 r.t_done = t;
                 ready.push_back(r);
@@ -642,6 +725,19 @@ static Scanner *get_scanner(pgap_engine *e) {
 }
 
 extern "C" {
+
+pgap_status pgap_scan_side_skip(pgap_engine *e, int enable, float min_merit) {
+    if (!e) return PGAP_E_ARG;
+    if (!e->ready) return PGAP_E_STATE;
+    Scanner *s = get_scanner(e);
+    if (s->inited) {
+        e->set_error("pgap_scan_side_skip: must be called before the first submit");
+        return PGAP_E_STATE;
+    }
+    s->side_skip = enable != 0;
+    s->skip_min_merit = min_merit / 2 - 2;  // gap_test_gpu.cu:604, in float
+    return PGAP_OK;
+}
 
 pgap_status pgap_scan_config(pgap_engine *e, uint32_t depth, uint32_t confirm_rounds) {
     if (!e) return PGAP_E_ARG;

@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from typing import Iterable, Optional, Sequence
+from typing import Iterable, List, Optional, Sequence
 
 import numpy as np
 
@@ -29,6 +29,20 @@ SPRP = 1
 
 ROW_PREV_FALLBACK = 1
 ROW_NEXT_FALLBACK = 2
+ROW_NEXT_SKIPPED = 4
+
+def read_tuning(path: str) -> List[dict]:
+    """Parse an autotuner table (the text format pgap_autotune writes)."""
+    keys = ("test", "bits", "radix", "digits", "lanes", "cap", "tests_per_s")
+    rows = []
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#") or not line.strip():
+                continue
+            parts = line.split()
+            rows.append({k: (float(v) if k == "tests_per_s" else int(v)) for k, v in zip(keys, parts)})
+    return rows
+
 
 # Every symbol include/pgap.h declares; tests check the library exports them.
 EXPORTED = (
@@ -36,7 +50,7 @@ EXPORTED = (
     "pgap_prp_batch", "pgap_prp_batch_device", "pgap_prp_batch_mixed", "pgap_sync",
     "pgap_last_launch", "pgap_family", "pgap_scan_submit", "pgap_scan_poll",
     "pgap_scan_in_flight", "pgap_scan_config", "pgap_build_candidates_device", "pgap_filter_candidates",
-    "pgap_sieve", "pgap_scan_stats",
+    "pgap_sieve", "pgap_scan_stats", "pgap_autotune", "pgap_scan_side_skip",
 )
 
 
@@ -124,10 +138,12 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.pgap_scan_in_flight.argtypes = [vp]
     lib.pgap_scan_in_flight.restype = sz
     lib.pgap_scan_config.argtypes = [vp, u32, u32]
+    lib.pgap_scan_side_skip.argtypes = [vp, ctypes.c_int, ctypes.c_float]
     lib.pgap_build_candidates_device.argtypes = [vp, ctypes.c_int, vp, vp, sz, u32, vp]
     lib.pgap_filter_candidates.argtypes = [vp, ctypes.c_int, vp, vp, sz, u32, vp]
     lib.pgap_sieve.argtypes = [vp, ctypes.c_int, vp, sz, u64, vp, vp, vp, vp]
     lib.pgap_scan_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u32)]
+    lib.pgap_autotune.argtypes = [vp, ctypes.POINTER(u32), ctypes.c_size_t, ctypes.c_char_p]
     for name in EXPORTED:
         f = getattr(lib, name)
         if name not in ("pgap_destroy", "pgap_last_error", "pgap_version", "pgap_scan_in_flight"):
@@ -232,6 +248,18 @@ class Engine:
                                                   out.ctypes.data))
         return out
 
+    def prp_mixed_packed(self, words: np.ndarray, offset: np.ndarray, limbs: np.ndarray, kind: int = FERMAT2,
+                         bases: Sequence[int] = ()) -> np.ndarray:
+        """Number i = words[offset[i] : offset[i] + limbs[i]] (uint32 words)."""
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        o = np.ascontiguousarray(offset, dtype=np.uint64)
+        lb = np.ascontiguousarray(limbs, dtype=np.uint32)
+        out = np.zeros(len(lb), dtype=np.uint8)
+        b = np.ascontiguousarray(bases, dtype=np.uint32)
+        self._check(self.lib.pgap_prp_batch_mixed(self.h, w.ctypes.data, o.ctypes.data, lb.ctypes.data, len(lb), kind,
+                                                  b.ctypes.data if len(b) else None, len(b), out.ctypes.data))
+        return out
+
     def sync(self):
         self._check(self.lib.pgap_sync(self.h))
 
@@ -276,6 +304,10 @@ class Engine:
     def scan_config(self, depth: int, confirm_rounds: int):
         self._check(self.lib.pgap_scan_config(self.h, depth, confirm_rounds))
 
+    def scan_side_skip(self, enable: bool, min_merit: float = 18.0):
+        """gap_test_gpu's prev-first order with side-skip below min_merit/2 - 2."""
+        self._check(self.lib.pgap_scan_side_skip(self.h, 1 if enable else 0, float(min_merit)))
+
     def scan_submit(self, family: int, m: int, prev: np.ndarray, nxt: np.ndarray, tag: int = 0) -> int:
         """Returns PGAP_OK, or PGAP_E_OOM when admission is refused; raises otherwise."""
         p = np.ascontiguousarray(prev, dtype=np.int32)
@@ -295,6 +327,23 @@ class Engine:
 
     def scan_in_flight(self) -> int:
         return self.lib.pgap_scan_in_flight(self.h)
+
+    def autotune(self, widths: Sequence[int], path: Optional[str] = None) -> List[dict]:
+        """Time the kernel families on each width and keep the fastest per
+        (test, width); returns the table and writes it to `path` if given
+        (load it later with Engine(tuning_path=path))."""
+        arr = (ctypes.c_uint32 * len(widths))(*[int(w) for w in widths])
+        tmp = None
+        if path is None:
+            import tempfile
+            fd, tmp = tempfile.mkstemp(suffix=".tune")
+            os.close(fd)
+        try:
+            self._check(self.lib.pgap_autotune(self.h, arr, len(widths), (path or tmp).encode()))
+            return read_tuning(path or tmp)
+        finally:
+            if tmp:
+                os.unlink(tmp)
 
     def scan_stats(self) -> dict:
         t, r, s = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint32()

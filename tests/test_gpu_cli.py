@@ -88,3 +88,30 @@ def test_cli_results_file_and_mskip(tmp_path):
         _, p, n = lines[int(m)]
         want = ol.test_interval(907, 2190, int(m), 11000, p, n, kind=0)
         assert (int(prev_p), int(next_p)) == (want["prev_p"], want["next_p"])
+
+
+def test_cli_side_skip_rows_and_count():
+    # --side-skip: gap_test_gpu.cu:591-617 -- an m whose prev gap's merit is
+    # below min_merit/2 - 2 prints no row and counts as "only prev_prime".
+    min_merit = 6.0
+    r = subprocess.run([CLI, "-u", FIXTURE, "--min-merit", str(min_merit), "--side-skip"], capture_output=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr.decode()
+    out = r.stdout.decode()
+    K_log = ol.K_log(907, 2190)
+    thr = np.float32(np.float32(min_merit) / np.float32(2) - np.float32(2))
+    want_rows, skips = [], 0
+    for ln in open(FIXTURE, "rb").read().splitlines(keepends=True):
+        m, p, n = uf.parse_line(ln)
+        w = ol.test_interval(907, 2190, m, 11000, p, n, kind=0)
+        if np.float32(w["prev_p"] / (K_log + math.log(m))) < thr:
+            skips += 1
+            continue
+        gap = w["prev_p"] + w["next_p"]
+        merit = np.float32(gap / (K_log + math.log(m)))
+        if merit > np.float32(min_merit):
+            want_rows.append("%-5d %.4f  %d * 907#/2190 -%d to +%d" % (gap, merit, m, w["prev_p"], w["next_p"]))
+    got_rows = [l for l in out.splitlines() if re.match(r"^\d+\s+\d+\.\d{4}  \d+ \* ", l)]
+    assert got_rows == want_rows
+    assert 0 < skips
+    assert re.findall(r"only prev_prime (\d+) ", out)[-1] == str(skips)

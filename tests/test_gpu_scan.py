@@ -78,6 +78,53 @@ def test_alone_equals_mixed(unknowns_907):
         assert a == b, m
 
 
+@pytest.mark.parametrize("confirm", [0, 1, 2])
+def test_confirmation_rejects_fermat_pseudoprimes(confirm):
+    # K = 11#/1155 = 2 (init_K asserts K > 1), so N = 2m -+ x is small enough
+    # to place base-2 Fermat pseudoprimes in the lists: 404 - 63 = 341 =
+    # 11*31 and 404 + 157 = 561 = 3*11*17 (a Carmichael number).  The
+    # Fermat-only scan (the reference's GPU tester, ROUNDS=1) stops on them;
+    # with confirmation the hit fails base 3 and the side resumes past it, as
+    # mpz_probab_prime_p would.
+    P, D, SL, m = 11, 1155, 200, 202
+    prev = [63, 65, 67, 71]    # 341 (psp), 339 = 3*113, 337 prime, 333
+    nxt = [157, 159, 165]      # 561 (Carmichael), 563 prime, 569 prime
+    want = ol.test_interval(P, D, m, SL, prev, nxt, kind=0 if confirm else 1)
+    assert (want["prev_p"], want["next_p"]) == ((67, 159) if confirm else (63, 157))
+    with pgap.Engine(0) as e:
+        e.scan_config(4, confirm)
+        fam = e.family(P, D, SL)
+        rows = run_scan(e, fam, [(m, prev, nxt)])
+    r = rows[m]
+    assert (r.prev_p, r.next_p, r.p_tests, r.n_tests, r.flags) == \
+        (want["prev_p"], want["next_p"], want["p_tests"], want["n_tests"], 0)
+
+
+@pytest.mark.parametrize("min_merit", [6.0, 7.5])
+def test_side_skip_follows_gap_test_gpu_rule(unknowns_907, min_merit):
+    # gap_test_gpu.cu:591-617: with the prev PRP found and the next side not
+    # started, (float)(prev_p / (K_log + log m)) < min_merit/2 - 2 ends the m.
+    lines = lines_907(unknowns_907)
+    K_log = ol.K_log(907, 2190)
+    thr = np.float32(np.float32(min_merit) / np.float32(2) - np.float32(2))
+    with pgap.Engine(0) as e:
+        e.scan_side_skip(True, min_merit)
+        fam = e.family(907, 2190, 11000)
+        rows = run_scan(e, fam, lines)
+    skipped = 0
+    for m, p, n in lines:
+        want = ol.test_interval(907, 2190, m, 11000, p, n, kind=0)
+        r = rows[m]
+        skip = np.float32(want["prev_p"] / (K_log + math.log(m))) < thr
+        skipped += bool(skip)
+        if skip:
+            expect = (want["prev_p"], 0, want["p_tests"], 0, pgap.ROW_NEXT_SKIPPED)
+        else:
+            expect = (want["prev_p"], want["next_p"], want["p_tests"], want["n_tests"], 0)
+        assert (r.prev_p, r.next_p, r.p_tests, r.n_tests, r.flags) == expect, m
+    assert 0 < skipped < len(lines)
+
+
 def test_fallback_beyond_sieve_length():
     # README.md:40-46 record: 11244911 * 809#/1841070, next prime +17354 is
     # beyond SL = 15000, so the next side runs out of unknowns and continues
