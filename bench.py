@@ -292,11 +292,12 @@ def bench_stream(engine, pgap, n_m, load, seed, cpu_sample, threads):
     out = {
         "workload": f"P={P4} d={D4} SL={SL4}, m in seeded random order from [1,200000) coprime to d, unknowns "
                     f"sieved to 1e9 ({np.mean([len(p) + len(q) for p, q in lists]):.0f} per m), Fermat-2 scan "
-                    f"with 2 strong-PRP confirmations, depth <= 8",
+                    f"with 2 strong-PRP confirmations, per-round depth sized to the resident lanes (cap 256)",
         "m": n_m,
         "capacity_m_per_s": cap_mps,
         "useful_tests_per_s": useful / cap_s,
         "issued_tests_per_s": (s1["tests_issued"] - s0["tests_issued"]) / cap_s,
+        "rounds": s1["rounds"] - s0["rounds"],
         "offered_load": load,
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
@@ -328,7 +329,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--count", type=int, default=1 << 20)
     ap.add_argument("--seed", type=int, default=20261202 + 1)
-    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--cpu-sample", type=int, default=393216,
+                    help="candidates the CPU baseline tests (about 10 s on 16 host threads)")
+    ap.add_argument("--stream-cpu-sample", type=int, default=768,
+                    help="m's the CPU scan baseline runs (about 10 s on 16 host threads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-2180", action="store_true", help="skip the 2180-bit batch")
     ap.add_argument("--no-stream", action="store_true", help="skip the serving measurement")
@@ -403,6 +407,14 @@ def main():
     w_container = 2 * 40 * 40 * e_mean   # the 1280-bit (40-limb) container
 
     verd = d_out.to_host()
+    # the same batch through pgap_prp_batch (host buffers in and out, PCIe
+    # included) -- reported beside `value`, never as it
+    host_n = d_n.to_host()
+    engine.prp(host_n, pgap.FERMAT2)
+    t_h = time.perf_counter()
+    host_verd = engine.prp(host_n, pgap.FERMAT2)
+    pcie_rate = args.count / (time.perf_counter() - t_h)
+    del host_n
     result = {
         "metric": METRIC,
         "value": value,
@@ -437,6 +449,8 @@ def main():
             "kernel_ms_avg": avg_kernel_s * 1e3,
         },
         "prp_found": int(verd.sum()),
+        "host_buffer_tests_per_s": pcie_rate,
+        "host_buffer_verdicts_identical": bool((host_verd == verd).all()),
         "setup_s": round(setup_s, 2),
     }
     if not args.no_cpu:
@@ -463,7 +477,7 @@ def main():
     if not args.no_stream:
         threads = min(16, os.cpu_count() or 1)
         extra["stream"] = bench_stream(engine, pgap, args.stream_m, args.stream_load, args.seed + 2,
-                                       0 if args.no_cpu else 64, threads)
+                                       0 if args.no_cpu else args.stream_cpu_sample, threads)
     if extra:
         result["extra"] = extra
     print(json.dumps(result), flush=True)

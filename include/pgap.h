@@ -165,10 +165,15 @@ typedef struct {
 } pgap_scan_row;
 
 /* Submit one m.  PGAP_E_OOM means the HBM budget has no room right now:
- * poll, then retry (admission control). */
+ * poll, then retry (admission control).  The first submit starts the
+ * engine's scan worker (a host thread with its own stream) that runs rounds
+ * while anything is in flight; m's submitted while others are on the device
+ * join the next round.  Submit and poll may be called from one client thread
+ * (not concurrently from several). */
 pgap_status pgap_scan_submit(pgap_engine *e, const pgap_scan_req *req);
-/* Drive the device and collect finished rows (up to cap).  wait != 0 blocks
- * until at least one row is ready or nothing is in flight. */
+/* Collect finished rows (up to cap).  wait != 0 blocks until at least one
+ * row is ready or nothing is in flight.  An error inside the worker is
+ * returned by the next poll or submit. */
 pgap_status pgap_scan_poll(pgap_engine *e, pgap_scan_row *rows, size_t cap, size_t *n_out, int wait);
 /* Number of m's submitted and not yet returned. */
 size_t pgap_scan_in_flight(const pgap_engine *e);
@@ -198,10 +203,11 @@ pgap_status pgap_autotune(pgap_engine *e, const uint32_t *bits, size_t n, const 
  * before the first submit. */
 pgap_status pgap_scan_side_skip(pgap_engine *e, int enable, float min_merit);
 
-/* Streaming knobs: speculative depth per side (tests in flight per side),
- * Miller-Rabin confirmation rounds on each side's first PRP (bases 3, 5, ...),
- * and the confirmation's effect: a found PRP failing confirmation is treated
- * as composite and the scan continues. */
+/* Streaming knobs: the cap on speculative depth per side (a round gives
+ * each unfinished side about resident-lanes / sides tests, never more than
+ * this; default 256, at most 4096), and the Miller-Rabin confirmation rounds
+ * on each side's first Fermat PRP (bases 3, 5, ...; default 2): a PRP failing
+ * confirmation is treated as composite and the scan continues past it. */
 pgap_status pgap_scan_config(pgap_engine *e, uint32_t depth, uint32_t confirm_rounds);
 
 #ifdef __cplusplus

@@ -105,34 +105,38 @@ pgap_status pgap_engine::ensure(DevBuf &b, size_t bytes, const char *what) {
     return PGAP_OK;
 }
 
-pgap_status pgap_engine::pick(int kind, uint32_t bits, const Variant **out) {
-    const Variant *v = nullptr;
+Choice pgap_engine::select(int kind, uint32_t bits) const {
+    Choice c;
+    std::lock_guard<std::mutex> g(tune_mu);
     const TuneEntry *t = (opt.radix || opt.lanes_per_number) ? nullptr : tuned(kind, bits);
-    if (t) v = find_variant(kind, (int)bits, t->radix, t->lanes);
-    if (!v) v = find_variant(kind, (int)bits, opt.radix, opt.lanes_per_number);
-    if (!v && (opt.radix || opt.lanes_per_number)) v = find_variant(kind, (int)bits, 0, 0);
-    if (!v) {
+    if (t) c.v = find_variant(kind, (int)bits, t->radix, t->lanes);
+    if (!c.v) c.v = find_variant(kind, (int)bits, opt.radix, opt.lanes_per_number);
+    if (!c.v && (opt.radix || opt.lanes_per_number)) c.v = find_variant(kind, (int)bits, 0, 0);
+    if (!c.v) return c;
+    if (opt.waves_per_simd > 0)
+        c.cap = opt.waves_per_simd;
+    else if (t && t->radix == c.v->radix && t->lanes == c.v->lanes && t->digits == c.v->digits)
+        c.cap = t->blocks_cap;
+    return c;
+}
+
+pgap_status pgap_engine::pick(int kind, uint32_t bits, Choice *out) {
+    *out = select(kind, bits);
+    if (!out->v) {
         set_error("no kernel variant for %u-bit numbers (kind %d)", bits, kind);
         return PGAP_E_TOO_LARGE;
     }
-    *out = v;
-    cur_cap = cap_for(v, bits);
     return PGAP_OK;
 }
 
-int pgap_engine::cap_for(const Variant *v, uint32_t bits) const {
-    if (opt.waves_per_simd > 0) return opt.waves_per_simd;
-    const TuneEntry *t = tuned(v->kind, bits);
-    return (t && t->radix == v->radix && t->lanes == v->lanes && t->digits == v->digits) ? t->blocks_cap : 0;
-}
-
-pgap_status pgap_engine::launch_batch(const Variant *v, const uint32_t *d_n, uint32_t limbs, size_t count,
+pgap_status pgap_engine::launch_batch(const Choice &c, const uint32_t *d_n, uint32_t limbs, size_t count,
                                       const Bases &bases, uint8_t *d_out) {
+    const Variant *v = c.v;
     hipError_t e = hipEventRecord(ev0, stream);
     if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
     // Grid dimensions are 32-bit: split very large batches.
     const size_t chunk = (size_t)1 << 30;
-    const int cap = cur_cap;
+    const int cap = c.cap;
     uint32_t launches = 0;
     for (size_t off = 0; off < count; off += chunk) {
         const uint32_t c = (uint32_t)std::min(chunk, count - off);
@@ -287,7 +291,7 @@ pgap_status pgap_prp_batch(pgap_engine *e, uint32_t limbs, const uint32_t *n, si
     if (st) return st;
     (void)hipSetDevice(e->opt.device);
     const uint32_t bits = host_max_bits(n, limbs, count);
-    const Variant *v;
+    Choice v;
     if ((st = e->pick((int)kind, std::max(bits, 2u), &v))) return st;
     const size_t in_bytes = count * limbs * sizeof(uint32_t);
     if ((st = e->ensure(e->d_n, in_bytes, "numbers")) || (st = e->ensure(e->d_out, count, "verdicts")))
@@ -331,7 +335,7 @@ pgap_status pgap_prp_batch_device(pgap_engine *e, uint32_t limbs, const uint32_t
         if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
         if (he != hipSuccess) return e->hip_fail(he, "max bits");
     }
-    const Variant *v;
+    Choice v;
     if ((st = e->pick((int)kind, std::max(max_bits, 2u), &v))) return st;
     return e->launch_batch(v, d_n, limbs, count, bs, d_verdict);
 }

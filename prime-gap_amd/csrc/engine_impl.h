@@ -29,6 +29,12 @@ struct TuneEntry {
     double rate = 0;
 };
 
+// A variant and the occupancy cap to launch it with.
+struct Choice {
+    const Variant *v = nullptr;
+    int cap = 0;
+};
+
 // K = P#/d with what the device needs to build and screen m*K + x.
 struct Family {
     uint32_t P = 0, D = 0, SL = 0;
@@ -52,10 +58,13 @@ struct pgap_engine {
     pgap_options opt{};
     bool ready = false;
     int num_cus = 256;  // compute units of the device (queried at create)
-    std::vector<pgap::TuneEntry> tune;  // autotuner table (sorted by kind, bits)
-    int cur_cap = 0;                    // occupancy cap of the variant last picked
-    int cap_for(const pgap::Variant *v, uint32_t bits) const;    bool load_tuning(const char *path);
-    const pgap::TuneEntry *tuned(int kind, uint32_t bits) const;
+    // autotuner table (sorted by kind, bits); tune_mu guards it against a
+    // pgap_autotune call while the scan worker picks variants
+    std::vector<pgap::TuneEntry> tune;
+    mutable std::mutex tune_mu;
+    bool load_tuning(const char *path);
+    const pgap::TuneEntry *tuned(int kind, uint32_t bits) const;  // caller holds tune_mu
+    pgap::Choice select(int kind, uint32_t bits) const;             // variant + occupancy cap, or none
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timing_pending = false;
@@ -65,7 +74,7 @@ struct pgap_engine {
     std::string err;
 
     pgap::DevBuf d_n, d_out, d_small, d_m, d_x, d_keep, d_idx0, d_idx1;
-    std::vector<pgap::Family> families;
+    std::vector<pgap::Family> families;  // appended only by pgap_family (client thread)
     pgap::Scanner *scanner = nullptr;  // owned; released by scan_shutdown()
 
     // primes <= sieve_limit (unordered) with Barrett constants, for pgap_sieve
@@ -83,9 +92,8 @@ struct pgap_engine {
     void set_error(const char *fmt, ...) __attribute__((format(printf, 2, 3)));
     pgap_status hip_fail(hipError_t e, const char *what);
     pgap_status ensure(pgap::DevBuf &b, size_t bytes, const char *what);
-    pgap_status pick(int kind, uint32_t bits, const pgap::Variant **out);
-    // launches on the variant last returned by pick(), with its occupancy cap
-    pgap_status launch_batch(const pgap::Variant *v, const uint32_t *d_n, uint32_t limbs, size_t count,
+    pgap_status pick(int kind, uint32_t bits, pgap::Choice *out);  // select(), error when none covers
+    pgap_status launch_batch(const pgap::Choice &c, const uint32_t *d_n, uint32_t limbs, size_t count,
                              const pgap::Bases &bases, uint8_t *d_out);
     void collect_timing();
     void scan_shutdown();

@@ -47,7 +47,7 @@ struct Args {
     float min_merit = 18;
     uint64_t mskip = 0;
     int verbose = 2;
-    uint32_t depth = 8, confirm = 2;
+    uint32_t depth = 256, confirm = 2;
     size_t hbm_budget = 0;
     std::string results;
     bool dump = false;
@@ -61,7 +61,7 @@ void usage(const char *name) {
     printf("    (or -p <p> -d <d> --mstart <m> --minc <n> --sieve-length <SL> --max-prime <MP>)\n\n");
     printf("[OPTIONALLY]\n  --min-merit <x>   only display gaps with merit > x (default 18)\n");
     printf("  --mskip <m>       skip m < mskip (partial resume)\n");
-    printf("  --depth <n>       speculative tests per side per round (default 8)\n");
+    printf("  --depth <n>       cap on speculative tests per side per round (default 256)\n");
     printf("  --confirm <n>     strong-PRP bases confirming each side's first PRP (default 2)\n");
     printf("  --hbm-budget-mb <n>  device memory the scan may keep resident\n");
     printf("  --side-skip       gap_test_gpu order: prev side first, skip next if prev merit < min/2-2\n");

@@ -63,6 +63,7 @@ struct Variant {
     batch_launch_fn launch;
     int (*blocks_per_cu)();  // resident 256-thread workgroups per CU (occupancy query)
     scan_launch_fn scan = nullptr;  // set on strong-PRP variants (max_bits is the SPRP limit)
+    int (*scan_blocks_per_cu)() = nullptr;  // occupancy of the scan kernel
 };
 
 // Registry: filled by every instantiation unit at library load.

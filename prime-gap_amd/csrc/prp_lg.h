@@ -32,17 +32,23 @@ __device__ __forceinline__ uint32_t bitlen(const uint32_t (&d)[S], uint32_t g) {
     return qmax(b);
 }
 
-// one = R mod N, two = 2R mod N by modular doublings from 2^(b-1).
+// one = R mod N by modular doublings from 2^(b-1) (26*4S - (b-1) of them).
+template <int S>
+__device__ __forceinline__ void mont_one(const uint32_t (&n)[S], uint32_t b, uint32_t g, uint32_t (&one)[S]) {
+    const uint32_t top = b - 1, td = top / RB, tb = top % RB;
+#pragma unroll
+    for (int j = 0; j < S; ++j) one[j] = (g * S + (uint32_t)j == td) ? (1u << tb) : 0u;
+    const uint32_t cnt = (uint32_t)RB * 4 * S - top;
+    for (uint32_t c = 0; c < cnt; ++c) mod_double<S>(one, n, g);
+}
+
+// one = R mod N, two = 2R mod N.
 template <int S>
 __device__ __forceinline__ void mont_one_two(const uint32_t (&n)[S], uint32_t b, uint32_t g, uint32_t (&one)[S],
                                              uint32_t (&two)[S]) {
-    const uint32_t top = b - 1, td = top / RB, tb = top % RB;
+    mont_one<S>(n, b, g, one);
 #pragma unroll
-    for (int j = 0; j < S; ++j) two[j] = (g * S + (uint32_t)j == td) ? (1u << tb) : 0u;
-    const uint32_t cnt = (uint32_t)RB * 4 * S - top;
-    for (uint32_t c = 0; c < cnt; ++c) mod_double<S>(two, n, g);
-#pragma unroll
-    for (int j = 0; j < S; ++j) one[j] = two[j];
+    for (int j = 0; j < S; ++j) two[j] = one[j];
     mod_double<S>(two, n, g);
 }
 
@@ -55,8 +61,10 @@ __device__ __forceinline__ uint32_t fermat2(const uint32_t *__restrict__ nw, uin
     const uint32_t b = bitlen<S>(n, g);
     if (b < 2) return 0;
     const uint32_t np = neg_inv(n0);
-    uint32_t one[S], x[S];
-    mont_one_two<S>(n, b, g, one, x);
+    // x = Mont(2); Mont(1) is rebuilt after the loop instead of held
+    uint32_t x[S];
+    mont_one<S>(n, b, g, x);
+    mod_double<S>(x, n, g);
     int i = (int)b - 2;
     uint32_t ew = nw[i >> 5];
     while (i >= 0) {
@@ -69,6 +77,8 @@ __device__ __forceinline__ uint32_t fermat2(const uint32_t *__restrict__ nw, uin
         --i;
     }
     cond_sub<S>(x, n, g);
+    uint32_t one[S];
+    mont_one<S>(n, b, g, one);
     return equal<S>(x, one) ? 1u : 0u;
 }
 

@@ -77,6 +77,13 @@ hipError_t launch_scan_lg(const uint32_t *d_n, uint32_t limbs, uint32_t conf_row
     return hipGetLastError();
 }
 
+template <int S>
+int occupancy_scan_lg() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_lg_kernel<S>, 256, 0) != hipSuccess || n <= 0) n = 1;
+    return n;
+}
+
 template <int S, int KIND>
 int occupancy_lg() {
     int n = 0;
@@ -92,7 +99,7 @@ struct RegisterLG {
                                   &occupancy_lg<Ss, KIND_FERMAT2>}),
          ...);
         (register_variant(Variant{lg::RB, 4 * Ss, 4, KIND_SPRP, lg::RB * 4 * Ss - 14, &launch_lg<Ss, KIND_SPRP>,
-                                  &occupancy_lg<Ss, KIND_SPRP>, &launch_scan_lg<Ss>}),
+                                  &occupancy_lg<Ss, KIND_SPRP>, &launch_scan_lg<Ss>, &occupancy_scan_lg<Ss>}),
          ...);
     }
 };

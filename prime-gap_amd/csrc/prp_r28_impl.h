@@ -76,6 +76,13 @@ hipError_t launch_scan_r28(const uint32_t *d_n, uint32_t limbs, uint32_t conf_ro
     return hipGetLastError();
 }
 
+template <int L>
+int occupancy_scan_r28() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_r28_kernel<L>, 256, 0) != hipSuccess || n <= 0) n = 1;
+    return n;
+}
+
 template <int L, int KIND>
 int occupancy_r28() {
     int n = 0;
@@ -91,7 +98,7 @@ struct RegisterR28 {
                                   &occupancy_r28<Ls, KIND_FERMAT2>}),
          ...);
         (register_variant(Variant{28, Ls, 1, KIND_SPRP, 28 * Ls - 14, &launch_r28<Ls, KIND_SPRP>,
-                                  &occupancy_r28<Ls, KIND_SPRP>, &launch_scan_r28<Ls>}),
+                                  &occupancy_r28<Ls, KIND_SPRP>, &launch_scan_r28<Ls>, &occupancy_scan_r28<Ls>}),
          ...);
     }
 };

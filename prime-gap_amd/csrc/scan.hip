@@ -28,13 +28,19 @@
 // have no factor below 2^14 -- the prevprime(centre - SL) / nextprime(centre
 // + SL) fallback of gap_test_common.cpp:160-178, here on the GPU.
 // New requests are admitted between rounds as slots and budget allow, so m's
-// that arrive while others are on the device join the next round.
+// that arrive while others are on the device join the next round.  Rounds run
+// on an engine-owned worker thread with its own stream from the first submit
+// on; submit and poll only exchange requests and rows with it.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "engine_impl.h"
@@ -44,7 +50,7 @@ namespace pgap {
 constexpr uint32_t NONE = 0xffffffffu;
 constexpr uint32_t FALLBACK_CHUNK = 4096;  // candidates appended per exhausted side
 constexpr uint32_t FALLBACK_SPAN = 65536;  // offsets screened per append
-constexpr uint32_t MAX_SLOTS = 1u << 14;  // enough m's in flight to fill the device at depth 2
+constexpr uint32_t MAX_SLOTS = 1u << 16;  // m's in flight (the HBM budget may allow fewer)
 
 enum : uint32_t {
     F_PREV_DONE = 1,
@@ -134,22 +140,49 @@ __global__ void scatter_kernel(const Upload *__restrict__ up, const uint32_t *__
     }
 }
 
-// One thread per (active slot, side): a confirmation item for a pending
-// Fermat hit (rows [0, 2 na), counter ctr[3]) or up to `depth` Fermat items
-// (rows 2 na + [0, ctr[0])).
-__global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, SlotState *__restrict__ st,
-                             uint32_t depth, Bases conf, uint32_t *__restrict__ items,
-                             uint32_t *__restrict__ item_idx, uint8_t *__restrict__ item_base,
+// What a side does this round: 0 nothing, 1 a confirmation test, 2 Fermat
+// tests of its next unknowns.
+__device__ __forceinline__ uint32_t side_role(const SlotState &s, uint32_t side) {
+    const uint32_t done = side ? F_NEXT_DONE : F_PREV_DONE;
+    const uint32_t wait = side ? F_NEXT_WAIT : F_PREV_WAIT;
+    if (s.flags & (done | wait)) return 0;
+    if (side == 1 && (s.flags & (F_SEQ | F_PREV_DONE)) == F_SEQ) return 0;  // next waits for prev
+    if (s.cand[side] != NONE) return 1;
+    return s.cursor[side] < s.len[side] ? 2 : 0;
+}
+
+// ctr[1] = sides that run Fermat tests this round, ctr[4] = sides that run
+// a confirmation test (the Fermat depth divides the resident lanes the
+// confirmations leave among the scanning sides).
+__global__ void count_kernel(const uint32_t *__restrict__ active, uint32_t na, const SlotState *__restrict__ st,
                              uint32_t *__restrict__ ctr) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t role = t < 2 * na ? side_role(st[active[t >> 1]], t & 1u) : 0u;
+    const uint64_t bs = __ballot(role == 2), bc = __ballot(role == 1);
+    if ((threadIdx.x & 63u) == 0) {
+        if (bs) atomicAdd(&ctr[1], (uint32_t)__popcll(bs));
+        if (bc) atomicAdd(&ctr[4], (uint32_t)__popcll(bc));
+    }
+}
+
+// One thread per (active slot, side): a confirmation item for a pending
+// Fermat hit (rows [0, 2 na), counter ctr[3]) or Fermat items (rows
+// 2 na + [0, ctr[0])).  Each scanning side takes an equal share of the
+// resident lanes the confirmations (padded to whole wavefronts of `cpad`
+// items) leave, at least one, capped by `depth` and by the `maxi` rows the
+// round holds: a grid larger than the resident lanes would run a second,
+// mostly empty wave and double the round's time.
+__global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, SlotState *__restrict__ st,
+                             uint32_t depth, uint32_t resident, uint32_t cpad, uint32_t maxi, Bases conf,
+                             uint32_t *__restrict__ items, uint32_t *__restrict__ item_idx,
+                             uint8_t *__restrict__ item_base, uint32_t *__restrict__ ctr) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= 2 * na) return;
     const uint32_t slot = active[t >> 1], side = t & 1u;
     SlotState &s = st[slot];
-    const uint32_t done = side ? F_NEXT_DONE : F_PREV_DONE;
-    const uint32_t wait = side ? F_NEXT_WAIT : F_PREV_WAIT;
-    if (s.flags & (done | wait)) return;
-    if (side == 1 && (s.flags & (F_SEQ | F_PREV_DONE)) == F_SEQ) return;  // next waits for prev
-    if (s.cand[side] != NONE) {
+    const uint32_t role = side_role(s, side);
+    if (role == 0) return;
+    if (role == 1) {
         const uint32_t r = atomicAdd(&ctr[3], 1u);
         items[r] = (slot << 1) | side;
         item_idx[r] = s.cand[side];
@@ -157,8 +190,11 @@ __global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, S
         return;
     }
     const uint32_t c = s.cursor[side], l = s.len[side];
-    if (c >= l) return;
-    const uint32_t k = min(depth, l - c);
+    const uint32_t ns = max(ctr[1], 1u);
+    const uint32_t used = (ctr[4] + cpad - 1) / cpad * cpad;
+    const uint32_t room = resident > used ? resident - used : 0u;
+    const uint32_t d = min(min(depth, max(1u, room / ns)), max(1u, maxi / ns));
+    const uint32_t k = min(d, l - c);
     const uint32_t base = 2 * na + atomicAdd(&ctr[0], k);
     for (uint32_t j = 0; j < k; ++j) {
         items[base + j] = (slot << 1) | side;
@@ -272,9 +308,20 @@ static double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// What the worker needs of a family, copied at the family's first submit so
+// the worker never reads the engine's family table (the client thread may
+// append to it at any time).
+struct FamInfo {
+    int id = 0;
+    Words K;
+    uint32_t SL = 0;
+    double K_log = 0;
+    std::vector<uint32_t> kmod;  // K mod p over the fallback primes (worker, lazily)
+};
+
 struct SlotHost {
     bool used = false;
-    int family = 0;
+    FamInfo *fam = nullptr;
     uint64_t m = 0, tag = 0;
     uint32_t orig_len[2] = {0, 0};
     uint32_t len[2] = {0, 0};
@@ -285,7 +332,7 @@ struct SlotHost {
 };
 
 struct Staged {
-    int family;
+    FamInfo *fam;
     uint64_t m, tag;
     std::vector<int32_t> prev, next;
     double t_submit;
@@ -294,23 +341,35 @@ struct Staged {
 class Scanner {
   public:
     pgap_engine *e;
-    uint32_t depth = 32, confirm_rounds = 2;
+    uint32_t depth = 256, confirm_rounds = 2;
     bool side_skip = false;       // gap_test_gpu's prev-first order with side-skip
     float skip_min_merit = 0;     // min_merit / 2 - 2 (float, gap_test_gpu.cu:604)
     uint32_t S = 0, CW = 0, CAP = 0, W = 0, MAXI = 0;
     bool inited = false;
 
-    std::mutex mu;                 // guards `staged` (submit thread vs poll thread)
+    // Shared with the client thread, under mu: staged requests, ready rows,
+    // the admitted count, the family copies, telemetry and the worker's
+    // status.  Everything else belongs to the worker alone.
+    std::mutex mu;
+    std::condition_variable cv_work;   // worker: requests arrived / stop
+    std::condition_variable cv_ready;  // client: rows ready / worker stopped
     std::deque<Staged> staged;
+    std::vector<pgap_scan_row> ready;
+    size_t admitted = 0;               // m's in slots
+    std::map<int, FamInfo> fams;       // node addresses stay valid across inserts
+    pgap_status worker_status = PGAP_OK;
+    std::string worker_err;
+    bool stop = false;
+    std::thread worker;
+    hipStream_t stream = nullptr;      // the worker's stream
+
     std::vector<SlotHost> slots;
     std::vector<uint32_t> free_slots, active;
-    std::vector<pgap_scan_row> ready;
     std::vector<Upload> pending_append;      // fallback appends for the next round
     std::vector<std::vector<uint32_t>> pending_append_data;
 
-    // family small-prime residues for the fallback screen
+    // small primes for the fallback screen
     std::vector<uint32_t> fb_primes;
-    std::vector<std::vector<uint32_t>> fb_kmod;  // per family
 
     SlotState *d_state = nullptr;
     uint32_t *d_center = nullptr, *d_active = nullptr, *d_items = nullptr, *d_idx = nullptr, *d_nbuf = nullptr,
@@ -327,13 +386,26 @@ class Scanner {
     Upload *h_up = nullptr;
     uint32_t *h_updata = nullptr;
 
-    uint64_t tests_issued = 0, rounds = 0;
+    uint64_t tests_issued = 0, rounds = 0;  // under mu
 
     explicit Scanner(pgap_engine *eng) : e(eng) {}
-    ~Scanner() { release(); }
+    ~Scanner() {
+        shutdown();
+        release();
+    }
+
+    // Stop the worker (it finishes the round in progress) and wait for it.
+    void shutdown() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            stop = true;
+        }
+        cv_work.notify_all();
+        if (worker.joinable()) worker.join();
+    }
 
     void release() {
-        (void)hipStreamSynchronize(e->stream);
+        if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : {(void *)d_state, (void *)d_center, (void *)d_active, (void *)d_items, (void *)d_idx,
                         (void *)d_nbuf, (void *)d_base, (void *)d_ctr, (void *)d_lists, (void *)d_verdict,
                         (void *)d_events, (void *)d_up, (void *)d_updata})
@@ -349,12 +421,38 @@ class Scanner {
         h_active = h_ctr = h_updata = nullptr;
         h_events = nullptr;
         h_up = nullptr;
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
         inited = false;
     }
 
+    size_t in_flight_locked() const { return staged.size() + admitted + ready.size(); }
     size_t in_flight() {
         std::lock_guard<std::mutex> g(mu);
-        return staged.size() + (active.size()) + ready.size();
+        return in_flight_locked();
+    }
+
+    void worker_main() {
+        (void)hipSetDevice(e->opt.device);
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stop || !staged.empty() || admitted > 0; });
+                if (stop) return;
+            }
+            const pgap_status st = round();
+            if (st != PGAP_OK) {
+                std::lock_guard<std::mutex> g(mu);
+                worker_status = st;
+                worker_err = pgap_last_error(e);
+                cv_ready.notify_all();
+                return;
+            }
+        }
+    }
+
+    void start_worker() {
+        if (!worker.joinable()) worker = std::thread([this] { worker_main(); });
     }
 
     template <class T>
@@ -393,40 +491,58 @@ class Scanner {
         CW = kw + 2;   // m < 2^64
         W = CW + 1;    // room for the carry of + x
         CAP = std::max<uint32_t>(sl, 1) + 4 * FALLBACK_CHUNK;
-        // per side: `depth` Fermat rows and one confirmation row (number,
-        // item, index, verdict; base for the confirmation row)
-        const size_t per_slot = sizeof(SlotState) + 4ull * CW + 8ull * CAP + 8 /*active*/ + sizeof(Event) +
-                                2ull * (depth_max() + 1) * (4ull * W + 4 + 4 + 1) + 2;
+        // A round carries about as many Fermat tests as the chosen variant
+        // keeps resident (the depth rule in round()), plus at most one per
+        // side when depth 1 already exceeds that: the row buffer is sized by
+        // the largest resident count over the variants plus two rows per
+        // slot, not by slots x depth.
+        size_t resident_max = 0;
+        for (size_t i = 0; i < variant_count(); ++i) {
+            const Variant *v = variant_at(i);
+            const size_t bpc = (size_t)std::max(1, v->blocks_per_cu ? v->blocks_per_cu() : 1);
+            resident_max = std::max(resident_max, bpc * (size_t)e->num_cus * 256 / (size_t)std::max(1, v->lanes));
+        }
+        const size_t row_bytes = 4ull * W + 4 + 4 + 1;  // number, item, index, verdict
+        // per slot: state, centre, lists, active entry, event, and per side
+        // one confirmation row (+ its base) and one Fermat row
+        const size_t per_slot = sizeof(SlotState) + 4ull * CW + 8ull * CAP + 4 + sizeof(Event) +
+                                2ull * (2 * row_bytes + 1);
         size_t budget = e->opt.hbm_budget;
         if (budget == 0) {
             size_t fr = 0, tot = 0;
             if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)8 << 30;
             budget = fr / 4;
         }
-        S = (uint32_t)std::min<size_t>(MAX_SLOTS, budget / per_slot);
+        // speculation rows take at most half of the budget
+        const size_t extra = std::min(resident_max, budget / 2 / row_bytes);
+        S = (uint32_t)std::min<size_t>(MAX_SLOTS, (budget - extra * row_bytes) / per_slot);
         if (S == 0) {
             e->set_error("scan: HBM budget %zu bytes cannot hold one m (%zu bytes per slot)", budget, per_slot);
             return PGAP_E_TOO_LARGE;
         }
-        MAXI = 2 * S * depth_max();
+        MAXI = (uint32_t)std::min<size_t>(extra + 2ull * S, 0x7fffffffu);
         const size_t rows = (size_t)2 * S + MAXI;  // confirmation rows, then Fermat rows
         pgap_status st;
         if ((st = dalloc(&d_state, S, "state")) || (st = dalloc(&d_center, (size_t)S * CW, "centres")) ||
             (st = dalloc(&d_lists, (size_t)S * 2 * CAP, "lists")) || (st = dalloc(&d_active, S, "active")) ||
             (st = dalloc(&d_items, rows, "items")) || (st = dalloc(&d_idx, rows, "item index")) ||
             (st = dalloc(&d_nbuf, rows * W, "numbers")) || (st = dalloc(&d_verdict, rows, "verdicts")) ||
-            (st = dalloc(&d_base, (size_t)2 * S, "confirmation bases")) || (st = dalloc(&d_ctr, 4, "counters")) ||
+            (st = dalloc(&d_base, (size_t)2 * S, "confirmation bases")) || (st = dalloc(&d_ctr, 8, "counters")) ||
             (st = dalloc(&d_events, S, "events")) || (st = halloc(&h_active, S, "active")) ||
             (st = halloc(&h_ctr, 4, "counters")) || (st = halloc(&h_events, S, "events"))) {
             release();
             return st;
+        }
+        hipError_t he = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+        if (he != hipSuccess) {
+            release();
+            return e->hip_fail(he, "scan stream");
         }
         slots.assign(S, SlotHost());
         free_slots.clear();
         for (uint32_t s = S; s-- > 0;) free_slots.push_back(s);
         // small primes for the beyond-SL screen (K mod p per family on demand)
         fb_primes = primes_upto(1u << 14);
-        fb_kmod.clear();
         inited = true;
         return PGAP_OK;
     }
@@ -444,14 +560,12 @@ class Scanner {
         return d > 4294967295.0 ? 0xffffffffu : (uint32_t)d;
     }
 
-    const std::vector<uint32_t> &kmod_small(int family) {
-        while (fb_kmod.size() <= (size_t)family) {
-            const Family &f = e->families[fb_kmod.size()];
-            std::vector<uint32_t> km(fb_primes.size());
-            for (size_t i = 0; i < fb_primes.size(); ++i) km[i] = mod_small(f.K, fb_primes[i]);
-            fb_kmod.push_back(std::move(km));
+    const std::vector<uint32_t> &kmod_small(FamInfo *f) {
+        if (f->kmod.empty()) {
+            f->kmod.resize(fb_primes.size());
+            for (size_t i = 0; i < fb_primes.size(); ++i) f->kmod[i] = mod_small(f->K, fb_primes[i]);
         }
-        return fb_kmod[(size_t)family];
+        return f->kmod;
     }
 
     uint32_t depth_max() const { return std::max<uint32_t>(depth, 1); }
@@ -486,7 +600,7 @@ class Scanner {
 
     // Screen offsets beyond SL for one exhausted side: no factor below 2^14.
     std::vector<uint32_t> fallback_candidates(SlotHost &sh, int side) {
-        const std::vector<uint32_t> &km = kmod_small(sh.family);
+        const std::vector<uint32_t> &km = kmod_small(sh.fam);
         std::vector<uint32_t> out;
         uint32_t x = sh.fb_from[side];
         const uint32_t end = x + FALLBACK_SPAN;
@@ -515,6 +629,7 @@ class Scanner {
                 take.push_back(std::move(staged.front()));
                 staged.pop_front();
             }
+            admitted += take.size();  // no window where an m is neither staged nor admitted
         }
         const size_t nup = take.size() + pending_append.size();
         if (nup == 0) return PGAP_OK;
@@ -527,13 +642,13 @@ class Scanner {
         for (Staged &r : take) {
             const uint32_t slot = free_slots.back();
             free_slots.pop_back();
-            const Family &f = e->families[(size_t)r.family];
+            const FamInfo &f = *r.fam;
             Words c = f.K;
             mul_small(c, r.m);
             SlotHost &sh = slots[slot];
             sh = SlotHost();
             sh.used = true;
-            sh.family = r.family;
+            sh.fam = r.fam;
             sh.m = r.m;
             sh.tag = r.tag;
             sh.orig_len[0] = sh.len[0] = (uint32_t)r.prev.size();
@@ -574,12 +689,12 @@ class Scanner {
         pending_append.clear();
         pending_append_data.clear();
         hipError_t he;
-        if ((he = hipMemcpyAsync(d_up, h_up, nup * sizeof(Upload), hipMemcpyHostToDevice, e->stream)) !=
+        if ((he = hipMemcpyAsync(d_up, h_up, nup * sizeof(Upload), hipMemcpyHostToDevice, stream)) !=
                 hipSuccess ||
             (he = hipMemcpyAsync(d_updata, h_updata, std::max<size_t>(off, 1) * 4, hipMemcpyHostToDevice,
-                                 e->stream)) != hipSuccess)
+                                 stream)) != hipSuccess)
             return e->hip_fail(he, "scan upload");
-        hipLaunchKernelGGL(scatter_kernel, dim3((uint32_t)nup), dim3(256), 0, e->stream, d_up, d_updata,
+        hipLaunchKernelGGL(scatter_kernel, dim3((uint32_t)nup), dim3(256), 0, stream, d_up, d_updata,
                            (uint32_t)nup, d_state, d_center, CW, d_lists, CAP);
         if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "scatter kernel");
         return PGAP_OK;
@@ -593,25 +708,25 @@ class Scanner {
         uint32_t bits = 0;
         for (uint32_t s : active) bits = std::max(bits, slots[s].bits);
         bits += 1;  // + x
-        const Variant *vf, *vc = nullptr;
-        if ((st = e->pick(KIND_FERMAT2, bits, &vf))) return st;
-        if (confirm_rounds && (st = e->pick(KIND_SPRP, bits, &vc))) return st;
+        Choice cf, cc;
+        if ((st = e->pick(KIND_FERMAT2, bits, &cf))) return st;
+        if (confirm_rounds && (st = e->pick(KIND_SPRP, bits, &cc))) return st;
+        const Variant *vf = cf.v, *vc = cc.v;
         if (vc && !vc->scan) {
             e->set_error("scan: variant radix %d x %d has no scan kernel", vc->radix, vc->digits);
             return PGAP_E_STATE;
         }
-        const int cap_f = e->cap_for(vf, bits), cap_c = vc ? e->cap_for(vc, bits) : 0;
+        const int cap_f = cf.cap, cap_c = cc.cap;
         // Speculative depth.  A round costs about one test's latency whether
         // it carries one item or a full device's worth (the kernel is VALU
         // bound per lane), so every round is sized to the numbers the chosen
         // variant keeps resident at once: few active sides get a deep
         // speculative window, many get one or two tests each.
         const Variant *vr = vc ? vc : vf;
+        int (*occ)() = vc ? vc->scan_blocks_per_cu : vf->blocks_per_cu;
         const uint32_t resident =
-            (uint32_t)std::max(1, vr->blocks_per_cu ? vr->blocks_per_cu() : 1) * e->num_cus * 256u /
-            (uint32_t)std::max(1, vr->lanes);
-        const uint32_t d =
-            std::max<uint32_t>(1, std::min<uint32_t>(depth_max(), (resident + 2 * na - 1) / (2 * na)));
+            (uint32_t)std::max(1, occ ? occ() : 1) * e->num_cus * 256u / (uint32_t)std::max(1, vr->lanes);
+        const uint32_t cpad = 64u / (uint32_t)std::max(1, vr->lanes);  // items per wavefront
         Bases none{}, conf{};
         static const uint32_t mr_bases[kMaxBases] = {3, 5, 7, 11, 13, 17, 19, 23};
         conf.n = std::min<uint32_t>(confirm_rounds, kMaxBases);
@@ -619,41 +734,45 @@ class Scanner {
 
         memcpy(h_active, active.data(), na * 4);
         hipError_t he;
-        if ((he = hipMemcpyAsync(d_active, h_active, na * 4, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
-            (he = hipMemsetAsync(d_ctr, 0, 16, e->stream)) != hipSuccess)
+        if ((he = hipMemcpyAsync(d_active, h_active, na * 4, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+            (he = hipMemsetAsync(d_ctr, 0, 8 * sizeof(uint32_t), stream)) != hipSuccess)
             return e->hip_fail(he, "scan round set-up");
-        const uint32_t cr = 2 * na;     // confirmation rows (at most one per side)
-        const uint32_t maxi = 2 * na * d;  // Fermat rows
+        const uint32_t cr = 2 * na;  // confirmation rows (at most one per side)
+        // Fermat rows: the per-side depth (set on the device from the number
+        // of scanning sides) gives at most resident + one row per side
+        const uint32_t maxi = (uint32_t)std::min<uint64_t>(MAXI, (uint64_t)resident + 2ull * na);
         const uint32_t nrows = cr + maxi;
         const dim3 b256(256);
-        hipLaunchKernelGGL(issue_kernel, dim3((2 * na + 255) / 256), b256, 0, e->stream, d_active, na, d_state, d,
-                           conf, d_items, d_idx, d_base, d_ctr);
-        hipLaunchKernelGGL(materialize_kernel, dim3((nrows + 255) / 256), b256, 0, e->stream, d_items, d_idx, cr,
+        hipLaunchKernelGGL(count_kernel, dim3((2 * na + 255) / 256), b256, 0, stream, d_active, na, d_state,
+                           d_ctr);
+        hipLaunchKernelGGL(issue_kernel, dim3((2 * na + 255) / 256), b256, 0, stream, d_active, na, d_state,
+                           depth_max(), resident, cpad, maxi, conf, d_items, d_idx, d_base, d_ctr);
+        hipLaunchKernelGGL(materialize_kernel, dim3((nrows + 255) / 256), b256, 0, stream, d_items, d_idx, cr,
                            nrows, d_ctr, d_center, CW, d_lists, CAP, d_nbuf, W);
         if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "scan issue");
         if (vc)
-            he = vc->scan(d_nbuf, W, cr, cr, maxi, d_ctr, d_base, d_verdict, cap_c, e->stream);
+            he = vc->scan(d_nbuf, W, cr, cr, maxi, d_ctr, d_base, d_verdict, cap_c, stream);
         else
             he = vf->launch(d_nbuf + (size_t)cr * W, W, maxi, d_ctr, nullptr, none, d_verdict + cr, cap_f,
-                            e->stream);
+                            stream);
         if (he != hipSuccess) return e->hip_fail(he, "scan test launch");
-        hipLaunchKernelGGL(retire_kernel, dim3((nrows + 255) / 256), b256, 0, e->stream, d_items, d_idx, cr, nrows,
+        hipLaunchKernelGGL(retire_kernel, dim3((nrows + 255) / 256), b256, 0, stream, d_items, d_idx, cr, nrows,
                            d_verdict, d_ctr, confirm_rounds, d_state);
-        hipLaunchKernelGGL(update_kernel, dim3((na + 255) / 256), b256, 0, e->stream, d_active, na, d_state, d_lists,
+        hipLaunchKernelGGL(update_kernel, dim3((na + 255) / 256), b256, 0, stream, d_active, na, d_state, d_lists,
                            CAP, d_ctr, d_events);
         if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "scan retire");
-        if ((he = hipMemcpyAsync(h_ctr, d_ctr, 16, hipMemcpyDeviceToHost, e->stream)) != hipSuccess ||
-            (he = hipStreamSynchronize(e->stream)) != hipSuccess)
+        if ((he = hipMemcpyAsync(h_ctr, d_ctr, 16, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+            (he = hipStreamSynchronize(stream)) != hipSuccess)
             return e->hip_fail(he, "scan round");
         const uint32_t nev = h_ctr[2];
-        tests_issued += (uint64_t)h_ctr[0] + h_ctr[3];
-        ++rounds;
+        const uint64_t issued = (uint64_t)h_ctr[0] + h_ctr[3];
         if (nev) {
             if ((he = hipMemcpy(h_events, d_events, nev * sizeof(Event), hipMemcpyDeviceToHost)) != hipSuccess)
                 return e->hip_fail(he, "scan events");
         }
         const double t = now_s();
         std::vector<uint32_t> finished;
+        std::vector<pgap_scan_row> rows;
         for (uint32_t i = 0; i < nev; ++i) {
             const Event &ev = h_events[i];
             SlotHost &sh = slots[ev.slot];
@@ -681,7 +800,7 @@ class Scanner {
                 pgap_scan_row r{};
                 r.m = sh.m;
                 r.tag = sh.tag;
-                r.family = (uint32_t)sh.family;
+                r.family = (uint32_t)sh.fam->id;
                 r.prev_p = ev.x[0];
                 r.next_p = ev.x[1];
                 r.p_tests = std::min(ev.found[0] + 1, sh.orig_len[0]);
@@ -690,7 +809,7 @@ class Scanner {
                           ((ev.flags & F_NEXT_SKIP) ? PGAP_ROW_NEXT_SKIPPED : 0);
                 r.t_submit = sh.t_submit;
                 r.t_done = t;
-                ready.push_back(r);
+                rows.push_back(r);
                 finished.push_back(ev.slot);
             }
         }
@@ -706,6 +825,14 @@ class Scanner {
                 free_slots.push_back(s);
             }
         }
+        {
+            std::lock_guard<std::mutex> g(mu);
+            ready.insert(ready.end(), rows.begin(), rows.end());
+            admitted -= finished.size();
+            tests_issued += issued;
+            ++rounds;
+        }
+        if (!rows.empty()) cv_ready.notify_all();
         return PGAP_OK;
     }
 };
@@ -715,7 +842,7 @@ class Scanner {
 using namespace pgap;
 
 void pgap_engine::scan_shutdown() {
-    delete scanner;
+    delete scanner;  // stops and joins the worker first
     scanner = nullptr;
 }
 
@@ -747,8 +874,8 @@ pgap_status pgap_scan_config(pgap_engine *e, uint32_t depth, uint32_t confirm_ro
         e->set_error("pgap_scan_config: must be called before the first submit");
         return PGAP_E_STATE;
     }
-    if (depth == 0 || depth > 256 || confirm_rounds > (uint32_t)kMaxBases) {
-        e->set_error("pgap_scan_config: depth in [1,256], confirm_rounds in [0,%d]", kMaxBases);
+    if (depth == 0 || depth > 4096 || confirm_rounds > (uint32_t)kMaxBases) {
+        e->set_error("pgap_scan_config: depth in [1,4096], confirm_rounds in [0,%d]", kMaxBases);
         return PGAP_E_ARG;
     }
     s->depth = depth;
@@ -787,20 +914,38 @@ pgap_status pgap_scan_submit(pgap_engine *e, const pgap_scan_req *req) {
             return PGAP_E_ARG;
         }
     Staged r;
-    r.family = req->family;
     r.m = req->m;
     r.tag = req->tag;
     r.prev.assign(req->prev, req->prev + req->n_prev);
     r.next.assign(req->next, req->next + req->n_next);
     r.t_submit = now_s();
-    std::lock_guard<std::mutex> g(s->mu);
-    // admission control: staged requests wait for a free slot; refuse when the
-    // backlog already fills every slot once more
-    if (s->staged.size() >= s->S) {
-        e->set_error("pgap_scan_submit: HBM budget full (%u slots busy, %zu waiting)", s->S, s->staged.size());
-        return PGAP_E_OOM;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->worker_status != PGAP_OK) {
+            e->set_error("%s", s->worker_err.c_str());
+            return s->worker_status;
+        }
+        // admission control: staged requests wait for a free slot; refuse
+        // when the backlog already fills every slot once more
+        if (s->staged.size() >= s->S) {
+            e->set_error("pgap_scan_submit: HBM budget full (%u slots busy, %zu waiting)", s->S,
+                         s->staged.size());
+            return PGAP_E_OOM;
+        }
+        auto it = s->fams.find(req->family);
+        if (it == s->fams.end()) {
+            FamInfo fi;
+            fi.id = req->family;
+            fi.K = f->K;
+            fi.SL = f->SL;
+            fi.K_log = f->K_log;
+            it = s->fams.emplace(req->family, std::move(fi)).first;
+        }
+        r.fam = &it->second;
+        s->staged.push_back(std::move(r));
+        s->start_worker();
     }
-    s->staged.push_back(std::move(r));
+    s->cv_work.notify_one();
     return PGAP_OK;
 }
 
@@ -810,16 +955,14 @@ pgap_status pgap_scan_poll(pgap_engine *e, pgap_scan_row *rows, size_t cap, size
     *n_out = 0;
     Scanner *s = get_scanner(e);
     if (!s->inited) return PGAP_OK;
-    (void)hipSetDevice(e->opt.device);
-    for (;;) {
-        if (!s->ready.empty() || (s->active.empty() && s->in_flight() == 0)) break;
-        pgap_status st = s->round();
-        if (st) return st;
-        if (!wait) break;
-        if (s->active.empty() && s->pending_append.empty()) {
-            std::lock_guard<std::mutex> g(s->mu);
-            if (s->staged.empty()) break;
-        }
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (wait)
+        s->cv_ready.wait(lk, [&] {
+            return !s->ready.empty() || s->worker_status != PGAP_OK || s->in_flight_locked() == 0;
+        });
+    if (s->ready.empty() && s->worker_status != PGAP_OK) {
+        e->set_error("%s", s->worker_err.c_str());
+        return s->worker_status;
     }
     const size_t n = std::min(cap, s->ready.size());
     for (size_t i = 0; i < n; ++i) rows[i] = s->ready[i];
@@ -835,10 +978,18 @@ size_t pgap_scan_in_flight(const pgap_engine *e) {
 
 pgap_status pgap_scan_stats(const pgap_engine *e, uint64_t *tests_issued, uint64_t *rounds, uint32_t *slots) {
     if (!e) return PGAP_E_ARG;
-    const Scanner *s = e->scanner;
-    if (tests_issued) *tests_issued = s ? s->tests_issued : 0;
-    if (rounds) *rounds = s ? s->rounds : 0;
-    if (slots) *slots = s ? s->S : 0;
+    Scanner *s = e->scanner;
+    uint64_t t = 0, r = 0;
+    uint32_t n = 0;
+    if (s) {
+        std::lock_guard<std::mutex> g(s->mu);
+        t = s->tests_issued;
+        r = s->rounds;
+        n = s->S;
+    }
+    if (tests_issued) *tests_issued = t;
+    if (rounds) *rounds = r;
+    if (slots) *slots = n;
     return PGAP_OK;
 }
 

@@ -147,9 +147,10 @@ extern "C" pgap_status pgap_autotune(pgap_engine *e, const uint32_t *bits, size_
             fprintf(f, "%d %u %d %d %d %d %.1f\n", t.kind, t.bits, t.radix, t.digits, t.lanes, t.blocks_cap, t.rate);
         fclose(f);
     }
-    e->tune = found;
-    std::sort(e->tune.begin(), e->tune.end(), [](const TuneEntry &a, const TuneEntry &b) {
+    std::sort(found.begin(), found.end(), [](const TuneEntry &a, const TuneEntry &b) {
         return a.kind != b.kind ? a.kind < b.kind : a.bits < b.bits;
     });
+    std::lock_guard<std::mutex> g(e->tune_mu);
+    e->tune = found;
     return PGAP_OK;
 }
