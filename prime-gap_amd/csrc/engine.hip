@@ -129,6 +129,13 @@ pgap_status pgap_engine::pick(int kind, uint32_t bits, Choice *out) {
     return PGAP_OK;
 }
 
+// Rows whose verdict is 1: the numbers a later strong-PRP base still tests.
+__global__ void select_kernel(const uint8_t *__restrict__ verdict, uint32_t count, uint32_t *__restrict__ sel,
+                              uint32_t *__restrict__ nsel) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t < count && verdict[t]) sel[atomicAdd(nsel, 1u)] = t;
+}
+
 pgap_status pgap_engine::launch_batch(const Choice &c, const uint32_t *d_n, uint32_t limbs, size_t count,
                                       const Bases &bases, uint8_t *d_out) {
     const Variant *v = c.v;
@@ -138,11 +145,34 @@ pgap_status pgap_engine::launch_batch(const Choice &c, const uint32_t *d_n, uint
     const size_t chunk = (size_t)1 << 30;
     const int cap = c.cap;
     uint32_t launches = 0;
+    // Several strong-PRP bases: the first on every row, each later one only
+    // on the rows that passed so far (selected on the device), so lanes that
+    // already failed do not ride along through the later bases.
+    const bool staged = v->kind == KIND_SPRP && bases.n > 1 && count <= chunk;
+    Bases first = bases;
+    if (staged) first.n = 1;
     for (size_t off = 0; off < count; off += chunk) {
-        const uint32_t c = (uint32_t)std::min(chunk, count - off);
-        e = v->launch(d_n + off * limbs, limbs, c, nullptr, nullptr, bases, d_out + off, cap, stream);
+        const uint32_t n = (uint32_t)std::min(chunk, count - off);
+        e = v->launch(d_n + off * limbs, limbs, n, nullptr, nullptr, first, d_out + off, cap, stream);
         if (e != hipSuccess) return hip_fail(e, "kernel launch");
         ++launches;
+    }
+    if (staged) {
+        pgap_status st;
+        if ((st = ensure(d_sel, count * 4, "survivor index")) || (st = ensure(d_nsel, 64, "survivor count")))
+            return st;
+        uint32_t *sel = (uint32_t *)d_sel.p, *nsel = (uint32_t *)d_nsel.p;
+        for (uint32_t k = 1; k < bases.n; ++k) {
+            Bases one{};
+            one.n = 1;
+            one.b[0] = bases.b[k];
+            if ((e = hipMemsetAsync(nsel, 0, 4, stream)) != hipSuccess) return hip_fail(e, "survivor count");
+            hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0, stream, d_out,
+                               (uint32_t)count, sel, nsel);
+            e = v->launch(d_n, limbs, (uint32_t)count, nsel, sel, one, d_out, cap, stream);
+            if (e != hipSuccess) return hip_fail(e, "kernel launch");
+            ++launches;
+        }
     }
     e = hipEventRecord(ev1, stream);
     if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
@@ -261,7 +291,8 @@ void pgap_destroy(pgap_engine *e) {
         e->scan_shutdown();
         e->sieve_shutdown();
         e->families_shutdown();
-        for (DevBuf *b : {&e->d_n, &e->d_out, &e->d_small, &e->d_m, &e->d_x, &e->d_keep, &e->d_idx0, &e->d_idx1}) {
+        for (DevBuf *b : {&e->d_n, &e->d_out, &e->d_small, &e->d_m, &e->d_x, &e->d_keep, &e->d_idx0, &e->d_idx1,
+                          &e->d_sel, &e->d_nsel}) {
             if (b->p) (void)hipFree(b->p);
         }
         (void)hipEventDestroy(e->ev0);

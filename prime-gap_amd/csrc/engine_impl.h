@@ -74,6 +74,7 @@ struct pgap_engine {
     std::string err;
 
     pgap::DevBuf d_n, d_out, d_small, d_m, d_x, d_keep, d_idx0, d_idx1;
+    pgap::DevBuf d_sel, d_nsel;  // survivors between strong-PRP bases
     std::vector<pgap::Family> families;  // appended only by pgap_family (client thread)
     pgap::Scanner *scanner = nullptr;  // owned; released by scan_shutdown()
 

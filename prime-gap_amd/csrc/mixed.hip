@@ -8,9 +8,42 @@
 
 using namespace pgap;
 
+static pgap_status mixed_once(pgap_engine *e, const uint32_t *words, const uint64_t *offset, const uint32_t *limbs,
+                              size_t count, pgap_test kind, const uint32_t *bases, uint32_t nbases,
+                              uint8_t *verdict);
+
+// Several strong-PRP bases: the first base runs on every number, each later
+// base only on the numbers that passed so far (a few percent of sieved
+// candidates), so the widest lane of a wavefront never carries the others
+// through bases they have already failed.
 extern "C" pgap_status pgap_prp_batch_mixed(pgap_engine *e, const uint32_t *words, const uint64_t *offset,
                                             const uint32_t *limbs, size_t count, pgap_test kind,
                                             const uint32_t *bases, uint32_t nbases, uint8_t *verdict) {
+    if (!e || kind != PGAP_SPRP || nbases <= 1 || !bases || nbases > (uint32_t)kMaxBases || !verdict || !offset ||
+        !limbs)
+        return mixed_once(e, words, offset, limbs, count, kind, bases, nbases, verdict);
+    pgap_status st = mixed_once(e, words, offset, limbs, count, kind, bases, 1, verdict);
+    for (uint32_t k = 1; k < nbases && st == PGAP_OK; ++k) {
+        std::vector<uint64_t> off;
+        std::vector<uint32_t> lim;
+        std::vector<size_t> at;
+        for (size_t i = 0; i < count; ++i)
+            if (verdict[i]) {
+                off.push_back(offset[i]);
+                lim.push_back(limbs[i]);
+                at.push_back(i);
+            }
+        if (at.empty()) break;
+        std::vector<uint8_t> v(at.size());
+        st = mixed_once(e, words, off.data(), lim.data(), at.size(), kind, bases + k, 1, v.data());
+        for (size_t j = 0; j < at.size() && st == PGAP_OK; ++j) verdict[at[j]] = v[j];
+    }
+    return st;
+}
+
+static pgap_status mixed_once(pgap_engine *e, const uint32_t *words, const uint64_t *offset, const uint32_t *limbs,
+                              size_t count, pgap_test kind, const uint32_t *bases, uint32_t nbases,
+                              uint8_t *verdict) {
     if (!e) return PGAP_E_ARG;
     if (!e->ready) return PGAP_E_STATE;
     if (count == 0) return PGAP_OK;

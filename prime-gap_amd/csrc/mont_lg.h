@@ -1,22 +1,27 @@
-// mont_lg.h -- Montgomery arithmetic on numbers spread over a quad of lanes
-// (4 lanes per number), radix 2^26.
+// mont_lg.h -- Montgomery arithmetic on numbers spread over a group of G
+// lanes (G = 4 or 8 lanes per number), radix 2^26.
 //
 // Used where one lane cannot hold a number (above ~2240 bits the three
 // L-digit arrays of the one-lane kernel no longer fit the register file).
-// Lane g of a quad owns digits [g*S, (g+1)*S) of x and N; L = 4*S.
+// Lane g of a group owns digits [g*S, (g+1)*S) of x and N; L = G*S.
 //
 // Squaring is row-wise Montgomery (CIOS): for i = 0..L-1
 //     T += x_i * x;   m = T_0 * n' mod 2^26;   T += m * N;   T /= 2^26
-// x_i and m are broadcast inside the quad with DPP quad_perm (a modifier on
-// a v_mov, no LDS).  T is kept as 64-bit unnormalised digit accumulators; a
+// x_i and m are broadcast inside the group with DPP (a modifier on a v_mov,
+// no LDS): quad_perm within a quad, plus a row shift under a bank mask for
+// groups of 8.  T is kept as 64-bit unnormalised digit accumulators; a
 // 26x26-bit product is < 2^52 and each accumulator takes two per row for at
-// most L = 164 rows, < 2^61, so no carry moves until the row loop ends.  The
+// most L = 168 rows, < 2^61, so no carry moves until the row loop ends.  The
 // division by 2^26 moves every accumulator down one position; positions are
 // mapped to registers with a rotating offset (position p of row i lives in
 // register (i + p) mod S), so the shift costs one 64-bit DPP move from the
 // next lane, not S register moves.  The multiplier digits x_i come from a
 // copy of x that rotates one lane down after every S rows, so the broadcast
-// source is always lane 0 of the quad and the S-row body is unrolled once.
+// source is always lane 0 of the group and the S-row body is unrolled once.
+//
+// G = 8 halves the registers per lane (5S + temporaries), which lets a
+// 4247-bit number run at more than one wavefront per SIMD without the
+// accumulation registers spilling to AGPRs.
 //
 // Reference counterpart: CGBN's warp-cooperative cgbn_mont_sqr with
 // TPI threads per instance (miller_rabin.h:157-211, TPI = 8 in
@@ -33,6 +38,10 @@ constexpr uint32_t MASK = (1u << RB) - 1;
 
 __device__ __forceinline__ uint64_t mad(uint32_t a, uint32_t b, uint64_t c) { return (uint64_t)a * b + c; }
 
+// DPP control codes (GFX9 DPP16).
+constexpr int DPP_ROW_SHL1 = 0x101, DPP_ROW_SHL4 = 0x104, DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR4 = 0x114,
+              DPP_ROW_HALF_MIRROR = 0x141;
+
 // Lane g of quad receives lane SRC's value.
 template <int P0, int P1, int P2, int P3>
 __device__ __forceinline__ uint32_t qperm(uint32_t v) {
@@ -40,26 +49,60 @@ __device__ __forceinline__ uint32_t qperm(uint32_t v) {
 }
 template <int SRC>
 __device__ __forceinline__ uint32_t qbcast(uint32_t v) { return qperm<SRC, SRC, SRC, SRC>(v); }
-// lane g <- lane g+1 (lane 3 keeps its own value; callers mask it)
-__device__ __forceinline__ uint32_t qdown(uint32_t v) { return qperm<1, 2, 3, 3>(v); }
-// lane g <- lane g-1 (lane 0 keeps its own value; callers mask it)
-__device__ __forceinline__ uint32_t qup(uint32_t v) { return qperm<0, 0, 1, 2>(v); }
-__device__ __forceinline__ uint64_t qdown64(uint64_t v) {
-    return (uint64_t)qdown((uint32_t)v) | ((uint64_t)qdown((uint32_t)(v >> 32)) << 32);
-}
-__device__ __forceinline__ uint64_t qup64(uint64_t v) {
-    return (uint64_t)qup((uint32_t)v) | ((uint64_t)qup((uint32_t)(v >> 32)) << 32);
-}
-__device__ __forceinline__ uint32_t qmax(uint32_t v) {
-    v = max(v, qperm<1, 0, 3, 2>(v));
-    return max(v, qperm<2, 3, 0, 1>(v));
-}
-__device__ __forceinline__ uint32_t qor(uint32_t v) {
-    v |= qperm<1, 0, 3, 2>(v);
-    return v | qperm<2, 3, 0, 1>(v);
-}
 
-__device__ __forceinline__ uint32_t quad_lane() { return threadIdx.x & 3u; }
+// Group lane 0's value to every lane of the group.  For G = 8 the quad
+// broadcast is followed by a 4-lane shift written only to the upper quad of
+// each group (banks 1 and 3 of the 16-lane row).
+template <int G>
+__device__ __forceinline__ uint32_t gbcast0(uint32_t v) {
+    const uint32_t q = qbcast<0>(v);
+    if constexpr (G == 4) return q;
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)q, (int)q, DPP_ROW_SHR4, 0xF, 0xA, false);
+}
+// Group lane G-1's value to every lane of the group.
+template <int G>
+__device__ __forceinline__ uint32_t gbcast_top(uint32_t v) {
+    const uint32_t q = qbcast<3>(v);
+    if constexpr (G == 4) return q;
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)q, (int)q, DPP_ROW_SHL4, 0xF, 0x5, false);
+}
+// lane g <- lane g+1 (the top lane's value is undefined; callers mask it)
+template <int G>
+__device__ __forceinline__ uint32_t gdown(uint32_t v) {
+    if constexpr (G == 4) return qperm<1, 2, 3, 3>(v);
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_ROW_SHL1, 0xF, 0xF, false);
+}
+// lane g <- lane g-1 (lane 0's value is undefined; callers mask it)
+template <int G>
+__device__ __forceinline__ uint32_t gup(uint32_t v) {
+    if constexpr (G == 4) return qperm<0, 0, 1, 2>(v);
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_ROW_SHR1, 0xF, 0xF, false);
+}
+template <int G>
+__device__ __forceinline__ uint64_t gdown64(uint64_t v) {
+    return (uint64_t)gdown<G>((uint32_t)v) | ((uint64_t)gdown<G>((uint32_t)(v >> 32)) << 32);
+}
+template <int G>
+__device__ __forceinline__ uint64_t gup64(uint64_t v) {
+    return (uint64_t)gup<G>((uint32_t)v) | ((uint64_t)gup<G>((uint32_t)(v >> 32)) << 32);
+}
+// Reductions over the group: butterflies within the quad, then (G = 8) the
+// mirror of the 8-lane half-row, which pairs every lane with the other quad.
+template <int G>
+__device__ __forceinline__ uint32_t gmax(uint32_t v) {
+    v = max(v, qperm<1, 0, 3, 2>(v));
+    v = max(v, qperm<2, 3, 0, 1>(v));
+    if constexpr (G == 8)
+        v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_ROW_HALF_MIRROR, 0xF, 0xF, false));
+    return v;
+}
+template <int G>
+__device__ __forceinline__ uint32_t gor(uint32_t v) {
+    v |= qperm<1, 0, 3, 2>(v);
+    v |= qperm<2, 3, 0, 1>(v);
+    if constexpr (G == 8) v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_ROW_HALF_MIRROR, 0xF, 0xF, false);
+    return v;
+}
 
 __device__ __forceinline__ uint32_t neg_inv(uint32_t n0) {
     uint32_t x = n0;
@@ -71,8 +114,8 @@ __device__ __forceinline__ uint32_t neg_inv(uint32_t n0) {
 }
 
 // Carry-normalise unnormalised accumulators t (this lane's S positions) into
-// digits; the carry crosses lanes in three rounds.
-template <int S>
+// digits; the carry crosses lanes in G - 1 rounds.
+template <int S, int G>
 __device__ __forceinline__ void normalize(const uint64_t (&t)[S], uint32_t (&x)[S], uint32_t g) {
     uint64_t c = 0;
 #pragma unroll
@@ -82,8 +125,8 @@ __device__ __forceinline__ void normalize(const uint64_t (&t)[S], uint32_t (&x)[
         c = v >> RB;
     }
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        uint64_t cin = qup64(c);
+    for (int r = 0; r < G - 1; ++r) {
+        uint64_t cin = gup64<G>(c);
         if (g == 0) cin = 0;
         c = 0;
         uint64_t carry = cin;
@@ -97,9 +140,9 @@ __device__ __forceinline__ void normalize(const uint64_t (&t)[S], uint32_t (&x)[
     }
 }
 
-// y <- y - n over the quad; returns the final borrow (1 if y < n), valid in
+// y <- y - n over the group; returns the final borrow (1 if y < n), valid in
 // every lane.  Digits of y are left as y - n mod R (caller selects).
-template <int S>
+template <int S, int G>
 __device__ __forceinline__ uint32_t sub_n(const uint32_t (&y)[S], const uint32_t (&n)[S], uint32_t (&u)[S],
                                           uint32_t g) {
     uint32_t br = 0;
@@ -114,8 +157,8 @@ __device__ __forceinline__ uint32_t sub_n(const uint32_t (&y)[S], const uint32_t
     // make it borrow twice: only newly caused borrows travel on.
     uint32_t total = br, pending = br;
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        uint32_t bin = qup(pending);
+    for (int r = 0; r < G - 1; ++r) {
+        uint32_t bin = gup<G>(pending);
         if (g == 0) bin = 0;
         uint32_t b2 = bin;
 #pragma unroll
@@ -127,23 +170,23 @@ __device__ __forceinline__ uint32_t sub_n(const uint32_t (&y)[S], const uint32_t
         pending = b2;
         total |= b2;
     }
-    return qbcast<3>(total);
+    return gbcast_top<G>(total);
 }
 
 // y <- y mod N for y < 2N.
-template <int S>
+template <int S, int G>
 __device__ __forceinline__ void cond_sub(uint32_t (&y)[S], const uint32_t (&n)[S], uint32_t g) {
     uint32_t u[S];
-    const uint32_t borrow = sub_n<S>(y, n, u, g);
+    const uint32_t borrow = sub_n<S, G>(y, n, u, g);
 #pragma unroll
     for (int p = 0; p < S; ++p) y[p] = borrow ? y[p] : u[p];
 }
 
 // y <- y * 2^s (s in {0,1}), value stays below R.
-template <int S>
+template <int S, int G>
 __device__ __forceinline__ void shl(uint32_t (&y)[S], uint32_t s, uint32_t g) {
     uint32_t top = y[S - 1] >> (RB - 1);
-    uint32_t cin = qup(top);
+    uint32_t cin = gup<G>(top);
     if (g == 0) cin = 0;
 #pragma unroll
     for (int p = S - 1; p >= 1; --p) y[p] = ((y[p] << s) | ((y[p - 1] >> (RB - 1)) & s)) & MASK;
@@ -151,22 +194,22 @@ __device__ __forceinline__ void shl(uint32_t (&y)[S], uint32_t s, uint32_t g) {
 }
 
 // y <- 2y mod N for y < N.
-template <int S>
+template <int S, int G>
 __device__ __forceinline__ void mod_double(uint32_t (&y)[S], const uint32_t (&n)[S], uint32_t g) {
-    shl<S>(y, 1u, g);
-    cond_sub<S>(y, n, g);
+    shl<S, G>(y, 1u, g);
+    cond_sub<S, G>(y, n, g);
 }
 
-template <int S>
+template <int S, int G>
 __device__ __forceinline__ bool equal(const uint32_t (&a)[S], const uint32_t (&b)[S]) {
     uint32_t d = 0;
 #pragma unroll
     for (int p = 0; p < S; ++p) d |= a[p] ^ b[p];
-    return qor(d) == 0;
+    return gor<G>(d) == 0;
 }
 
-// x <- x^2 R^{-1} mod N (lazy: x < 4N in, < 2N out), R = 2^(26*4S).
-template <int S>
+// x <- x^2 R^{-1} mod N (lazy: x < 4N in, < 2N out), R = 2^(26*G*S).
+template <int S, int G>
 __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t np, uint32_t g) {
     uint64_t t[S];
     uint32_t xs[S];
@@ -175,26 +218,26 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S
         t[p] = 0;
         xs[p] = x[p];
     }
-    const bool lane0 = (g == 0), lane3 = (g == 3);
-    for (int o = 0; o < 4; ++o) {
+    const bool lane0 = (g == 0), top = (g == G - 1);
+    for (int o = 0; o < G; ++o) {
 #pragma unroll
         for (int u = 0; u < S; ++u) {
-            const uint32_t xi = qbcast<0>(xs[u]);
+            const uint32_t xi = gbcast0<G>(xs[u]);
 #pragma unroll
             for (int j = 0; j < S; ++j) t[(u + j) % S] = mad(xi, x[j], t[(u + j) % S]);
-            const uint32_t m = qbcast<0>(((uint32_t)t[u] * np) & MASK);
+            const uint32_t m = gbcast0<G>(((uint32_t)t[u] * np) & MASK);
 #pragma unroll
             for (int j = 0; j < S; ++j) t[(u + j) % S] = mad(m, n[j], t[(u + j) % S]);
-            const uint64_t in = qdown64(t[u]);
+            const uint64_t in = gdown64<G>(t[u]);
             const uint64_t c = lane0 ? (t[u] >> RB) : 0ull;
             t[(u + 1) % S] += c;
-            t[u] = lane3 ? 0ull : in;
+            t[u] = top ? 0ull : in;
         }
         // the next S rows broadcast the next lane's digits
 #pragma unroll
-        for (int u = 0; u < S; ++u) xs[u] = qdown(xs[u]);
+        for (int u = 0; u < S; ++u) xs[u] = gdown<G>(xs[u]);
     }
-    normalize<S>(t, x, g);
+    normalize<S, G>(t, x, g);
 }
 
 }  // namespace lg

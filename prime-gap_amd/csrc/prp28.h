@@ -116,9 +116,11 @@ __device__ __forceinline__ void mod_add(uint32_t (&a)[L], const uint32_t (&b)[L]
 }
 
 // Strong probable prime test to one base b < 64 (N - 1 = d * 2^tz):
-// x = b^d, pass if x = +-1 or x^(2^r) = -1 for some 0 < r < tz.  One loop
-// with a single squaring call site covers both the exponent bits of d and
-// the tz - 1 checking squarings, so the wide squaring is inlined once.
+// x = b^d, pass if x = +-1 or x^(2^r) = -1 for some 0 < r < tz.  Two
+// phases, each with its own squaring loop: the exponent bits of d, which
+// holds only x, N and the squaring's temporaries (so it runs as fast as the
+// Fermat loop), then at most tz - 1 checking squarings, the only place that
+// needs Mont(1) and Mont(-1).
 template <int L>
 __device__ __forceinline__ uint32_t sprp(const uint32_t* __restrict__ nw, uint32_t limbs, uint32_t base) {
     uint32_t n[L];
@@ -127,8 +129,6 @@ __device__ __forceinline__ uint32_t sprp(const uint32_t* __restrict__ nw, uint32
     const int b = bitlen<L>(n);
     if (b < 2) return 0;  // N = 1
     const uint32_t np = neg_inv28(n[0]);
-    uint32_t one[L], x[L];
-    mont_one_two<L>(n, b, one, x);  // x = Mont(2)
 
     int tz = 1;
     while (((nw[tz >> 5] >> (tz & 31)) & 1u) == 0) ++tz;
@@ -136,16 +136,54 @@ __device__ __forceinline__ uint32_t sprp(const uint32_t* __restrict__ nw, uint32
     uint32_t bb = base;
     if (b <= 32) bb = base % nw[0];  // N < 2^32: reduce the base
     if (bb == 0) return 0;           // x = 0 is never +-1
-    if (bb != 2) {
-        // Mont(bb) = bb * R mod N by double-and-add over the bits of bb.
+    uint32_t x[L];
+    {
+        // x = Mont(bb) = bb * R mod N: Mont(2) directly, else double-and-add
+        // over the bits of bb from Mont(1)
+        uint32_t one[L];
+        mont_one_two<L>(n, b, one, x);
+        if (bb != 2) {
 #pragma unroll
-        for (int j = 0; j < L; ++j) x[j] = 0;
-        for (int t = 5; t >= 0; --t) {
-            mod_double<L>(x, n);
-            if ((bb >> t) & 1u) mod_add<L>(x, one, n);
+            for (int j = 0; j < L; ++j) x[j] = 0;
+            for (int t = 5; t >= 0; --t) {
+                mod_double<L>(x, n);
+                if ((bb >> t) & 1u) mod_add<L>(x, one, n);
+            }
         }
     }
-    uint32_t mone[L];  // Mont(-1) = N - Mont(1)
+
+    // Phase 1: exponent bits of d below its top bit, positions b-2 .. tz of
+    // N - 1.  Base 2 folds the multiply into the squaring; other bases
+    // multiply by bb (x < 2*bb*N afterwards, which the next squaring
+    // absorbs: see mul_small).
+    int i = b - 2;
+    uint32_t ew = i >= tz ? nw[i >> 5] : 0u;
+    while (i >= tz) {
+        const uint32_t bit = (ew >> (i & 31)) & 1u;
+        uint32_t nxt = ew;
+        if ((i & 31) == 0 && i > tz) nxt = nw[(i >> 5) - 1];
+        sqr_step<L>(x, n, np, bb == 2 ? bit : 0u);
+        if (bb != 2 && bit) mul_small<L>(x, bb);
+        ew = nxt;
+        --i;
+    }
+
+    // Phase 2: x = Mont(b^d) < 2*bb*N.  Bring it below N, then compare with
+    // Mont(+-1) and square at most tz - 1 times looking for Mont(-1).
+    for (int r = 0; r < 128; ++r) {
+        uint32_t br = 0, u[L];
+#pragma unroll
+        for (int j = 0; j < L; ++j) {
+            uint32_t v = x[j] - n[j] - br;
+            br = v >> 31;
+            u[j] = v & MASK;
+        }
+        if (br) break;
+#pragma unroll
+        for (int j = 0; j < L; ++j) x[j] = u[j];
+    }
+    uint32_t one[L], mone[L];  // Mont(1), Mont(-1) = N - Mont(1)
+    mont_one<L>(n, b, one);
     {
         uint32_t br = 0;
 #pragma unroll
@@ -155,46 +193,14 @@ __device__ __forceinline__ uint32_t sprp(const uint32_t* __restrict__ nw, uint32
             mone[j] = v & MASK;
         }
     }
-
-    // Exponent bits of d below its top bit: positions b-2 .. tz of N - 1.
-    int i = b - 2;
-    int checks = 0;  // checking squarings done
-    uint32_t ew = i >= tz ? nw[i >> 5] : 0u;
-    bool have_base_power = (i < tz);  // x already equals Mont(b^d)
-    for (;;) {
-        uint32_t s = 0, bit = 0, nxt = ew;
-        const bool checking = have_base_power;
-        if (checking) {
-            // x < 2*bb*N here: bring it below N, then compare.
-            for (int r = 0; r < 128; ++r) {
-                uint32_t br = 0, u[L];
-#pragma unroll
-                for (int j = 0; j < L; ++j) {
-                    uint32_t v = x[j] - n[j] - br;
-                    br = v >> 31;
-                    u[j] = v & MASK;
-                }
-                if (br) break;
-#pragma unroll
-                for (int j = 0; j < L; ++j) x[j] = u[j];
-            }
-            if (equal<L>(x, mone)) return 1;
-            if (equal<L>(x, one)) return checks == 0 ? 1u : 0u;
-            if (checks + 1 >= tz) return 0;
-            ++checks;
-        } else {
-            bit = (ew >> (i & 31)) & 1u;
-            if ((i & 31) == 0 && i > tz) nxt = nw[(i >> 5) - 1];
-            s = bb == 2 ? bit : 0u;
-        }
-        sqr_step<L>(x, n, np, s);  // the only squaring call site
-        if (!checking) {
-            if (bb != 2 && bit) mul_small<L>(x, bb);
-            ew = nxt;
-            --i;
-            have_base_power = (i < tz);
-        }
+    if (equal<L>(x, one) || equal<L>(x, mone)) return 1;
+    for (int c = 1; c < tz; ++c) {
+        sqr_step<L>(x, n, np, 0u);
+        cond_sub<L>(x, n);  // x < 2N -> x < N
+        if (equal<L>(x, mone)) return 1;
+        if (equal<L>(x, one)) return 0;  // 1 without passing -1: composite
     }
+    return 0;
 }
 
 }  // namespace r28

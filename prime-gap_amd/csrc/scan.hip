@@ -59,6 +59,8 @@ enum : uint32_t {
     F_NEXT_WAIT = 8,
     F_SEQ = 16,        // side-skip mode: the next side starts after the prev side's PRP
     F_NEXT_SKIP = 32,  // side-skip taken: the next side was not scanned
+    F_PREV_CONF = 64,  // confirmation rows of the pending hit are in this round
+    F_NEXT_CONF = 128,
 };
 
 struct SlotState {
@@ -68,7 +70,7 @@ struct SlotState {
     uint32_t flags;
     uint32_t cwords;
     uint32_t cand[2];   // first Fermat hit awaiting confirmation (NONE if none)
-    uint32_t stage[2];  // confirmation bases it has passed
+    uint32_t stage[2];  // confirmation bases it passed in the round testing it
     uint32_t skip_below;  // F_SEQ: a prev distance below this skips the next side
     uint32_t pad;
 };
@@ -140,8 +142,9 @@ __global__ void scatter_kernel(const Upload *__restrict__ up, const uint32_t *__
     }
 }
 
-// What a side does this round: 0 nothing, 1 a confirmation test, 2 Fermat
-// tests of its next unknowns.
+// What a side does this round: 0 nothing, 1 the confirmation tests of its
+// pending hit (every base at once, one row each), 2 Fermat tests of its next
+// unknowns.
 __device__ __forceinline__ uint32_t side_role(const SlotState &s, uint32_t side) {
     const uint32_t done = side ? F_NEXT_DONE : F_PREV_DONE;
     const uint32_t wait = side ? F_NEXT_WAIT : F_PREV_WAIT;
@@ -151,29 +154,29 @@ __device__ __forceinline__ uint32_t side_role(const SlotState &s, uint32_t side)
     return s.cursor[side] < s.len[side] ? 2 : 0;
 }
 
-// ctr[1] = sides that run Fermat tests this round, ctr[4] = sides that run
-// a confirmation test (the Fermat depth divides the resident lanes the
-// confirmations leave among the scanning sides).
+// ctr[1] = sides that run Fermat tests this round, ctr[4] = confirmation
+// rows (the Fermat depth divides the resident lanes the confirmations leave
+// among the scanning sides).
 __global__ void count_kernel(const uint32_t *__restrict__ active, uint32_t na, const SlotState *__restrict__ st,
-                             uint32_t *__restrict__ ctr) {
+                             uint32_t nconf, uint32_t *__restrict__ ctr) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     const uint32_t role = t < 2 * na ? side_role(st[active[t >> 1]], t & 1u) : 0u;
     const uint64_t bs = __ballot(role == 2), bc = __ballot(role == 1);
     if ((threadIdx.x & 63u) == 0) {
         if (bs) atomicAdd(&ctr[1], (uint32_t)__popcll(bs));
-        if (bc) atomicAdd(&ctr[4], (uint32_t)__popcll(bc));
+        if (bc) atomicAdd(&ctr[4], (uint32_t)__popcll(bc) * nconf);
     }
 }
 
-// One thread per (active slot, side): a confirmation item for a pending
-// Fermat hit (rows [0, 2 na), counter ctr[3]) or Fermat items (rows
-// 2 na + [0, ctr[0])).  Each scanning side takes an equal share of the
+// One thread per (active slot, side): `nconf` confirmation items for a
+// pending Fermat hit, one per base (rows [0, cr), counter ctr[3]), or Fermat
+// items (rows cr + [0, ctr[0])).  Each scanning side takes an equal share of the
 // resident lanes the confirmations (padded to whole wavefronts of `cpad`
 // items) leave, at least one, capped by `depth` and by the `maxi` rows the
 // round holds: a grid larger than the resident lanes would run a second,
 // mostly empty wave and double the round's time.
 __global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, SlotState *__restrict__ st,
-                             uint32_t depth, uint32_t resident, uint32_t cpad, uint32_t maxi, Bases conf,
+                             uint32_t cr, uint32_t depth, uint32_t resident, uint32_t cpad, uint32_t maxi, Bases conf,
                              uint32_t *__restrict__ items, uint32_t *__restrict__ item_idx,
                              uint8_t *__restrict__ item_base, uint32_t *__restrict__ ctr) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
@@ -183,10 +186,14 @@ __global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, S
     const uint32_t role = side_role(s, side);
     if (role == 0) return;
     if (role == 1) {
-        const uint32_t r = atomicAdd(&ctr[3], 1u);
-        items[r] = (slot << 1) | side;
-        item_idx[r] = s.cand[side];
-        item_base[r] = (uint8_t)conf.b[s.stage[side]];
+        const uint32_t r = atomicAdd(&ctr[3], conf.n);
+        for (uint32_t j = 0; j < conf.n; ++j) {
+            items[r + j] = (slot << 1) | side;
+            item_idx[r + j] = s.cand[side];
+            item_base[r + j] = (uint8_t)conf.b[j];
+        }
+        s.stage[side] = 0;
+        atomicOr(&s.flags, side ? F_NEXT_CONF : F_PREV_CONF);  // the other side's thread writes flags too
         return;
     }
     const uint32_t c = s.cursor[side], l = s.len[side];
@@ -195,7 +202,7 @@ __global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, S
     const uint32_t room = resident > used ? resident - used : 0u;
     const uint32_t d = min(min(depth, max(1u, room / ns)), max(1u, maxi / ns));
     const uint32_t k = min(d, l - c);
-    const uint32_t base = 2 * na + atomicAdd(&ctr[0], k);
+    const uint32_t base = cr + atomicAdd(&ctr[0], k);
     for (uint32_t j = 0; j < k; ++j) {
         items[base + j] = (slot << 1) | side;
         item_idx[base + j] = c + j;
@@ -237,10 +244,10 @@ __global__ void materialize_kernel(const uint32_t *__restrict__ items, const uin
     }
 }
 
-// Confirmation rows advance or drop their side's pending hit; Fermat hits
-// become the side's pending hit (or, with no confirmation rounds, its PRP).
-// A side has either one confirmation row or Fermat rows in a round, never
-// both, so only the Fermat hits need atomics.
+// Confirmation rows count the bases their side's pending hit passed (the
+// update decides); Fermat hits become the side's pending hit (or, with no
+// confirmation rounds, its PRP).  A side has either confirmation rows or
+// Fermat rows in a round, never both.
 __global__ void retire_kernel(const uint32_t *__restrict__ items, const uint32_t *__restrict__ item_idx,
                               uint32_t cr, uint32_t nrows, const uint8_t *__restrict__ verdict,
                               const uint32_t *__restrict__ ctr, uint32_t confirm_rounds,
@@ -250,13 +257,7 @@ __global__ void retire_kernel(const uint32_t *__restrict__ items, const uint32_t
     const uint32_t slot = items[t] >> 1, side = items[t] & 1u;
     SlotState &s = st[slot];
     if (t < cr) {
-        if (verdict[t]) {
-            if (++s.stage[side] >= confirm_rounds) s.found[side] = s.cand[side];
-        } else {
-            s.cursor[side] = s.cand[side] + 1;
-            s.cand[side] = NONE;
-            s.stage[side] = 0;
-        }
+        if (verdict[t]) atomicAdd(&s.stage[side], 1u);
         return;
     }
     if (!verdict[t]) return;
@@ -264,13 +265,27 @@ __global__ void retire_kernel(const uint32_t *__restrict__ items, const uint32_t
 }
 
 __global__ void update_kernel(const uint32_t *__restrict__ active, uint32_t na, SlotState *__restrict__ st,
-                              const int32_t *__restrict__ lists, uint32_t cap, uint32_t *__restrict__ ctr,
-                              Event *__restrict__ events) {
+                              const int32_t *__restrict__ lists, uint32_t cap, uint32_t nconf,
+                              uint32_t *__restrict__ ctr, Event *__restrict__ events) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= na) return;
     const uint32_t slot = active[t];
     SlotState &s = st[slot];
     uint32_t flags = s.flags, code = 0;
+    // a confirmation tested this round: every base passed -> the side's PRP;
+    // any failed (a base-2 pseudoprime) -> resume just past it
+    for (uint32_t side = 0; side < 2; ++side) {
+        const uint32_t conf = side ? F_NEXT_CONF : F_PREV_CONF;
+        if (!(flags & conf)) continue;
+        flags &= ~conf;
+        if (s.stage[side] >= nconf) {
+            s.found[side] = s.cand[side];
+        } else {
+            s.cursor[side] = s.cand[side] + 1;
+            s.cand[side] = NONE;
+        }
+        s.stage[side] = 0;
+    }
     for (uint32_t side = 0; side < 2; ++side) {
         const uint32_t done = side ? F_NEXT_DONE : F_PREV_DONE;
         const uint32_t wait = side ? F_NEXT_WAIT : F_PREV_WAIT;
@@ -505,8 +520,9 @@ class Scanner {
         const size_t row_bytes = 4ull * W + 4 + 4 + 1;  // number, item, index, verdict
         // per slot: state, centre, lists, active entry, event, and per side
         // one confirmation row (+ its base) and one Fermat row
+        const size_t nconf = std::max<uint32_t>(confirm_rounds, 1);
         const size_t per_slot = sizeof(SlotState) + 4ull * CW + 8ull * CAP + 4 + sizeof(Event) +
-                                2ull * (2 * row_bytes + 1);
+                                2ull * (nconf * (row_bytes + 1) + row_bytes);
         size_t budget = e->opt.hbm_budget;
         if (budget == 0) {
             size_t fr = 0, tot = 0;
@@ -521,13 +537,13 @@ class Scanner {
             return PGAP_E_TOO_LARGE;
         }
         MAXI = (uint32_t)std::min<size_t>(extra + 2ull * S, 0x7fffffffu);
-        const size_t rows = (size_t)2 * S + MAXI;  // confirmation rows, then Fermat rows
+        const size_t rows = 2ull * S * nconf + MAXI;  // confirmation rows, then Fermat rows
         pgap_status st;
         if ((st = dalloc(&d_state, S, "state")) || (st = dalloc(&d_center, (size_t)S * CW, "centres")) ||
             (st = dalloc(&d_lists, (size_t)S * 2 * CAP, "lists")) || (st = dalloc(&d_active, S, "active")) ||
             (st = dalloc(&d_items, rows, "items")) || (st = dalloc(&d_idx, rows, "item index")) ||
             (st = dalloc(&d_nbuf, rows * W, "numbers")) || (st = dalloc(&d_verdict, rows, "verdicts")) ||
-            (st = dalloc(&d_base, (size_t)2 * S, "confirmation bases")) || (st = dalloc(&d_ctr, 8, "counters")) ||
+            (st = dalloc(&d_base, 2ull * S * nconf, "confirmation bases")) || (st = dalloc(&d_ctr, 8, "counters")) ||
             (st = dalloc(&d_events, S, "events")) || (st = halloc(&h_active, S, "active")) ||
             (st = halloc(&h_ctr, 4, "counters")) || (st = halloc(&h_events, S, "events"))) {
             release();
@@ -737,15 +753,15 @@ class Scanner {
         if ((he = hipMemcpyAsync(d_active, h_active, na * 4, hipMemcpyHostToDevice, stream)) != hipSuccess ||
             (he = hipMemsetAsync(d_ctr, 0, 8 * sizeof(uint32_t), stream)) != hipSuccess)
             return e->hip_fail(he, "scan round set-up");
-        const uint32_t cr = 2 * na;  // confirmation rows (at most one per side)
+        const uint32_t cr = 2 * na * std::max<uint32_t>(conf.n, 1);  // confirmation rows: every base of a side
         // Fermat rows: the per-side depth (set on the device from the number
         // of scanning sides) gives at most resident + one row per side
         const uint32_t maxi = (uint32_t)std::min<uint64_t>(MAXI, (uint64_t)resident + 2ull * na);
         const uint32_t nrows = cr + maxi;
         const dim3 b256(256);
         hipLaunchKernelGGL(count_kernel, dim3((2 * na + 255) / 256), b256, 0, stream, d_active, na, d_state,
-                           d_ctr);
-        hipLaunchKernelGGL(issue_kernel, dim3((2 * na + 255) / 256), b256, 0, stream, d_active, na, d_state,
+                           conf.n, d_ctr);
+        hipLaunchKernelGGL(issue_kernel, dim3((2 * na + 255) / 256), b256, 0, stream, d_active, na, d_state, cr,
                            depth_max(), resident, cpad, maxi, conf, d_items, d_idx, d_base, d_ctr);
         hipLaunchKernelGGL(materialize_kernel, dim3((nrows + 255) / 256), b256, 0, stream, d_items, d_idx, cr,
                            nrows, d_ctr, d_center, CW, d_lists, CAP, d_nbuf, W);
@@ -759,7 +775,7 @@ class Scanner {
         hipLaunchKernelGGL(retire_kernel, dim3((nrows + 255) / 256), b256, 0, stream, d_items, d_idx, cr, nrows,
                            d_verdict, d_ctr, confirm_rounds, d_state);
         hipLaunchKernelGGL(update_kernel, dim3((na + 255) / 256), b256, 0, stream, d_active, na, d_state, d_lists,
-                           CAP, d_ctr, d_events);
+                           CAP, conf.n, d_ctr, d_events);
         if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "scan retire");
         if ((he = hipMemcpyAsync(h_ctr, d_ctr, 16, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
             (he = hipStreamSynchronize(stream)) != hipSuccess)

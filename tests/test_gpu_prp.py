@@ -96,7 +96,8 @@ def test_large_family_and_sprp(engine):
 
 @pytest.mark.parametrize("bits", [2300, 2490, 2600, 3000, 3320, 3500, 4100, 4250])
 def test_quad_variants_random_sizes(engine, bits):
-    # wider than one lane holds: four lanes per number
+    # wider than one lane holds: four lanes per number up to 3324 bits, eight
+    # above (the default choice where each measured faster)
     rng = random.Random(bits)
     vals = [rng.getrandbits(bits) | (1 << (bits - 1)) | 1 for _ in range(96)]
     p = []
@@ -107,7 +108,40 @@ def test_quad_variants_random_sizes(engine, bits):
         p.append(x)
     got = check(engine, vals + p)
     assert got[-2:].all()
-    assert engine.last_launch().lanes_per_number == 4
+    assert engine.last_launch().lanes_per_number == (4 if bits <= 3324 else 8)
+
+
+@pytest.fixture(scope="module")
+def engine8():
+    e = pgap.Engine(0, lanes=8)   # force the 8-lanes-per-number variants
+    yield e
+    e.close()
+
+
+@pytest.mark.parametrize("bits", [1500, 2128, 2280, 2300, 3000, 3320, 3500, 4100, 4250, 4300, 4360])
+def test_octet_variants_random_sizes(engine8, bits):
+    # eight lanes per number (radix 2^26, 21 digits per lane): up to 4364 bits
+    rng = random.Random(bits + 8)
+    vals = [rng.getrandbits(bits) | (1 << (bits - 1)) | 1 for _ in range(64)]
+    x = vals[0]
+    while not ol.probab_prime(x):
+        x += 2
+    got = check(engine8, vals + [x])
+    assert got[-1] == 1
+    assert engine8.last_launch().lanes_per_number == 8
+    if bits <= 26 * 168 - 14:
+        check(engine8, vals[:24] + [x], pgap.SPRP, (3, 5))
+        assert engine8.last_launch().lanes_per_number == 8
+
+
+def test_octet_variants_special(engine8):
+    vals = [(1 << 2281) - 1, (1 << 3217) - 1, (1 << 4253) - 1, (1 << 4096) + 1, (1 << 4253) + 2, (1 << 4363) - 1]
+    got = check(engine8, vals)
+    assert list(got[:4]) == [1, 1, 1, 1]
+    check(engine8, vals[:4], pgap.SPRP, (2,))
+    c = candidates(3001, 2190, 1, 100_000, 40000, 64, 18)   # 4231-4248-bit family
+    check(engine8, c)
+    check(engine8, c[:16], pgap.SPRP, (3, 5))
 
 
 def test_quad_variants_special_and_sprp(engine):
@@ -115,7 +149,7 @@ def test_quad_variants_special_and_sprp(engine):
             (1 << 3001) - 1]
     got = check(engine, vals)
     assert list(got[:4]) == [1, 1, 1, 1]
-    strong = [v for v in vals if v.bit_length() <= 26 * 164 - 14]  # widest strong-PRP variant
+    strong = [v for v in vals if v.bit_length() <= 26 * 168 - 14]  # widest strong-PRP variant
     check(engine, strong, pgap.SPRP, (2,))
     check(engine, strong, pgap.SPRP, (3, 5))
     c = candidates(3001, 2190, 1, 100_000, 40000, 64, 17)   # 4231-4248-bit family

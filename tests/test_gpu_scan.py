@@ -165,12 +165,13 @@ def test_scan_2100_bit_family_small():
             (want["prev_p"], want["next_p"], want["p_tests"], want["n_tests"]), m
 
 
-def test_scan_4300_bit_family_small():
-    # 3001#/2190 (N ~ 4231-4248 bits): quad-per-number variants in the scan
+@pytest.mark.parametrize("lanes", [0, 8])
+def test_scan_4300_bit_family_small(lanes):
+    # 3001#/2190 (N ~ 4231-4248 bits): lane-group variants in the scan
     P, D, SL = 3001, 2190, 40000
     sv = ol.Sieve(P, D, SL, 1_000_000)
     reqs = [(m,) + sv(m) for m in (1, 7)]
-    with pgap.Engine(0) as e:
+    with pgap.Engine(0, lanes=lanes) as e:
         fam = e.family(P, D, SL)
         rows = run_scan(e, fam, reqs)
     for m, p, n in reqs:
