@@ -242,18 +242,19 @@ def bench_batch_4300(engine, hipmem, pgap, count, seed):
     }
 
 
-def bench_stream(engine, pgap, n_m, load, seed, cpu_sample, threads):
+def bench_stream(engine, pgap, n_m, load, seed, cpu_sample, threads, P4=1511, D4=2190, SL4=40000, m_hi=200000,
+                 sieve_limit=1_000_000_000):
     """Serving: P=1511, d=2190, SL=40000, m's in seeded random order from
     [1, 200000) coprime to d, each with its unknowns sieved to 1e9 (device
     sieve, untimed).  Phase 1 saturates the engine (capacity); phase 2 feeds
-    Poisson arrivals at `load` x capacity and records arrival-to-row latency."""
-    P4, D4, SL4 = 1511, 2190, 40000
+    Poisson arrivals at `load` x capacity and records arrival-to-row latency.
+    The large-operand serving case passes P=3001, m < 1e5."""
     fam = engine.family(P4, D4, SL4)
     rng = np.random.default_rng(seed)
-    pool = np.array([m for m in range(1, 200000) if math.gcd(m, D4) == 1], dtype=np.uint64)
+    pool = np.array([m for m in range(1, m_hi) if math.gcd(m, D4) == 1], dtype=np.uint64)
     order = rng.permutation(pool)[: 2 * n_m]
     t0 = time.perf_counter()
-    lists = engine.sieve(fam, order, 1_000_000_000, SL4)
+    lists = engine.sieve(fam, order, sieve_limit, SL4)
     sieve_s = time.perf_counter() - t0
     cap_set, lat_set = list(range(n_m)), list(range(n_m, 2 * n_m))
 
@@ -290,8 +291,8 @@ def bench_stream(engine, pgap, n_m, load, seed, cpu_sample, threads):
     rows_lat, lat_s = run(lat_set, gaps)
     lat_ms = np.array([lat for _, lat in rows_lat.values()]) * 1e3
     out = {
-        "workload": f"P={P4} d={D4} SL={SL4}, m in seeded random order from [1,200000) coprime to d, unknowns "
-                    f"sieved to 1e9 ({np.mean([len(p) + len(q) for p, q in lists]):.0f} per m), Fermat-2 scan "
+        "workload": f"P={P4} d={D4} SL={SL4}, m in seeded random order from [1,{m_hi}) coprime to d, unknowns "
+                    f"sieved to {sieve_limit:.0e} ({np.mean([len(p) + len(q) for p, q in lists]):.0f} per m), Fermat-2 scan "
                     f"with 2 strong-PRP confirmations, per-round depth sized to the resident lanes (cap 256)",
         "m": n_m,
         "capacity_m_per_s": cap_mps,
@@ -336,6 +337,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-2180", action="store_true", help="skip the 2180-bit batch")
     ap.add_argument("--no-stream", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-stream-4300", action="store_true", help="skip the large-operand serving measurement")
+    ap.add_argument("--stream-4300-m", type=int, default=2048)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-width batch")
     ap.add_argument("--no-4300", action="store_true", help="skip the 4300-bit batch")
     ap.add_argument("--stream-m", type=int, default=16384)
@@ -478,6 +481,15 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         extra["stream"] = bench_stream(engine, pgap, args.stream_m, args.stream_load, args.seed + 2,
                                        0 if args.no_cpu else args.stream_cpu_sample, threads)
+    if not args.no_stream_4300:
+        # P=3001, d=2190, SL=40000, m < 1e5, unknowns sieved to 1e10
+        # (a fresh engine: a scan's slots are sized by the families it starts with)
+        engine2 = pgap.Engine(0)
+        threads = min(16, os.cpu_count() or 1)
+        extra["stream_4300_bit"] = bench_stream(engine2, pgap, args.stream_4300_m, args.stream_load, args.seed + 5,
+                                                0 if args.no_cpu else 16, threads, P4=3001, D4=2190, SL4=40000,
+                                                m_hi=100000, sieve_limit=10_000_000_000)
+        engine2.close()
     if extra:
         result["extra"] = extra
     print(json.dumps(result), flush=True)

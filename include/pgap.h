@@ -131,7 +131,8 @@ pgap_status pgap_filter_candidates(pgap_engine *e, int family, const uint64_t *m
  * m_i the prev distances go to prev + i*SL (increasing), the next distances
  * to next + i*SL, and their counts to n_prev[i], n_next[i] (host arrays).
  * The primes <= max_prime are generated on the device and kept for later
- * calls with the same limit.  max_prime <= 4e9. */
+ * calls with the same limit.  max_prime <= 2e10 (primes above 2^32 take
+ * 64-bit modular arithmetic; the table costs 24 bytes per prime of HBM). */
 pgap_status pgap_sieve(pgap_engine *e, int family, const uint64_t *m, size_t count, uint64_t max_prime,
                        int32_t *prev, uint32_t *n_prev, int32_t *next, uint32_t *n_next);
 

@@ -281,12 +281,19 @@ struct or_sieve {
     uint32_t P, D, SL;
     uint64_t max_prime;
     size_t nprimes;
-    uint32_t *primes;
-    uint32_t *kmod;   /* K mod p */
+    uint64_t *primes;
+    uint64_t *kmod;   /* K mod p */
 };
 
-or_sieve *or_sieve_new(uint32_t P, uint32_t D, uint32_t SL, uint64_t max_prime) {
-    if (max_prime > 4000000000ull || max_prime < 2) return NULL;
+/* a * b mod p for a, b < p < 2^63 (128-bit product) */
+static uint64_t mulmod64(uint64_t a, uint64_t b, uint64_t p) {
+    return (uint64_t)(((unsigned __int128)a * b) % p);
+}
+
+/* Primes in (lo, max_prime] with K mod p.  lo > 0 keeps only the upper
+ * range (a test of the primes above 2^32 need not pay for K mod p below). */
+or_sieve *or_sieve_new_range(uint32_t P, uint32_t D, uint32_t SL, uint64_t lo, uint64_t max_prime) {
+    if (max_prime > 20000000000ull || max_prime < 2) return NULL;
     mpz_t K;
     mpz_init(K);
     if (init_K(K, P, D) != 0) { mpz_clear(K); return NULL; }
@@ -298,24 +305,30 @@ or_sieve *or_sieve_new(uint32_t P, uint32_t D, uint32_t SL, uint64_t max_prime) 
         uint64_t p = 2 * i + 1;
         if (p > max_prime) break;
         cnt++;
-        for (uint64_t j = (p * p) / 2; j < half; j += p) comp[j >> 3] |= (uint8_t)(1u << (j & 7));
+        if (p <= max_prime / p)
+            for (uint64_t j = (p * p) / 2; j < half; j += p) comp[j >> 3] |= (uint8_t)(1u << (j & 7));
     }
     or_sieve *s = (or_sieve *)calloc(1, sizeof(or_sieve));
     s->P = P; s->D = D; s->SL = SL; s->max_prime = max_prime;
-    s->primes = (uint32_t *)malloc(sizeof(uint32_t) * cnt);
-    s->kmod = (uint32_t *)malloc(sizeof(uint32_t) * cnt);
+    s->primes = (uint64_t *)malloc(sizeof(uint64_t) * cnt);
+    s->kmod = (uint64_t *)malloc(sizeof(uint64_t) * cnt);
     size_t k = 0;
-    s->primes[k++] = 2;
+    if (lo < 2) s->primes[k++] = 2;
     for (uint64_t i = 1; i < half; i++) {
         uint64_t p = 2 * i + 1;
         if (p > max_prime) break;
-        if (!(comp[i >> 3] & (1u << (i & 7)))) s->primes[k++] = (uint32_t)p;
+        if (p > lo && !(comp[i >> 3] & (1u << (i & 7)))) s->primes[k++] = p;
     }
     s->nprimes = k;
-    for (size_t i = 0; i < k; i++) s->kmod[i] = (uint32_t)mpz_fdiv_ui(K, s->primes[i]);
+    /* K mod p for any p < 2^63: mpz_fdiv_ui takes an unsigned long (64-bit here) */
+    for (size_t i = 0; i < k; i++) s->kmod[i] = (uint64_t)mpz_fdiv_ui(K, (unsigned long)s->primes[i]);
     free(comp);
     mpz_clear(K);
     return s;
+}
+
+or_sieve *or_sieve_new(uint32_t P, uint32_t D, uint32_t SL, uint64_t max_prime) {
+    return or_sieve_new_range(P, D, SL, 0, max_prime);
 }
 
 void or_sieve_free(or_sieve *s) {
@@ -327,14 +340,42 @@ void or_sieve_free(or_sieve *s) {
 
 size_t or_sieve_nprimes(const or_sieve *s) { return s ? s->nprimes : 0; }
 
+/* For the offsets x in [-SL, SL] that some prime in (lo, max_prime] removes:
+ * x and the first such prime (in increasing order).  Returns the count, or
+ * -1 when more than cap.  Lets a test check the 64-bit arithmetic exactly:
+ * p must divide m*K + x. */
+int or_sieve_removed(const or_sieve *s, uint64_t m, uint64_t lo, int64_t *xs, uint64_t *ps, int cap) {
+    const uint32_t SL = s->SL;
+    const uint32_t W = 2 * SL + 1;
+    uint64_t *by = (uint64_t *)calloc(W, sizeof(uint64_t));
+    for (size_t i = 0; i < s->nprimes; i++) {
+        uint64_t p = s->primes[i];
+        if (p <= lo) continue;
+        uint64_t r = mulmod64(m % p, s->kmod[i], p);
+        uint64_t first = ((SL % p) + p - r) % p;
+        for (uint64_t j = first; j < W; j += p)
+            if (!by[j] || p < by[j]) by[j] = p;
+    }
+    int n = 0;
+    for (uint32_t j = 0; j < W; j++) {
+        if (!by[j]) continue;
+        if (n >= cap) { free(by); return -1; }
+        xs[n] = (int64_t)j - (int64_t)SL;
+        ps[n] = by[j];
+        n++;
+    }
+    free(by);
+    return n;
+}
+
 int or_sieve_m(const or_sieve *s, uint64_t m, int32_t *prev, int *np, int32_t *next, int *nn, int cap) {
     const uint32_t SL = s->SL;
     const uint32_t W = 2 * SL + 1;
     uint8_t *comp = (uint8_t *)calloc(W, 1);
     for (size_t i = 0; i < s->nprimes; i++) {
         uint64_t p = s->primes[i];
-        uint64_t r = (uint64_t)((m % p) * s->kmod[i] % p);       /* m*K mod p */
-        uint64_t first = ((SL % p) + p - r) % p;                   /* index of x = -r */
+        uint64_t r = mulmod64(m % p, s->kmod[i], p);                 /* m*K mod p */
+        uint64_t first = ((SL % p) + p - r) % p;                     /* index of x = -r */
         for (uint64_t j = first; j < W; j += p) comp[j] = 1;
     }
     comp[SL] = 1; /* x = 0 is never an unknown */

@@ -78,9 +78,11 @@ void or_test_interval(uint32_t P, uint32_t D, uint64_t m, uint32_t SL,
  * prev gets positive distances in increasing order, next likewise. */
 typedef struct or_sieve or_sieve;
 or_sieve *or_sieve_new(uint32_t P, uint32_t D, uint32_t SL, uint64_t max_prime);
+or_sieve *or_sieve_new_range(uint32_t P, uint32_t D, uint32_t SL, uint64_t lo, uint64_t max_prime);
 void or_sieve_free(or_sieve *s);
 int or_sieve_m(const or_sieve *s, uint64_t m, int32_t *prev, int *np, int32_t *next, int *nn, int cap);
 size_t or_sieve_nprimes(const or_sieve *s);
+int or_sieve_removed(const or_sieve *s, uint64_t m, uint64_t lo, int64_t *xs, uint64_t *ps, int cap);
 
 /* CPU baselines, timed by the caller: a static partition over `threads`
  * pthreads.  Return the number of items done.  kind as in or_verdicts. */

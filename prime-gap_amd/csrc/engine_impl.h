@@ -49,7 +49,7 @@ struct Family {
     uint64_t *d_mu = nullptr;    // floor((2^64-1)/p) for Barrett reduction
     uint32_t stage_end[2] = {0, 0};  // prime-index ends of the first two filter stages
     // K mod p over the engine's sieve prime table (pgap_sieve)
-    uint32_t *d_skmod = nullptr;
+    uint64_t *d_skmod = nullptr;
     uint64_t skmod_limit = 0;
 };
 }  // namespace pgap
@@ -79,7 +79,7 @@ struct pgap_engine {
     pgap::Scanner *scanner = nullptr;  // owned; released by scan_shutdown()
 
     // primes <= sieve_limit (unordered) with Barrett constants, for pgap_sieve
-    uint32_t *sieve_primes = nullptr;
+    uint64_t *sieve_primes = nullptr;
     uint64_t *sieve_mu = nullptr;
     size_t sieve_nprimes = 0;
     uint64_t sieve_limit = 0;

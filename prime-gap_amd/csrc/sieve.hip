@@ -24,11 +24,21 @@
 
 namespace pgap {
 
+// a mod p for any a < 2^64 and p < 2^62, mu = floor((2^64 - 1) / p): the
+// estimated quotient is short by at most 2.
 __device__ __forceinline__ uint64_t mod_b(uint64_t a, uint64_t p, uint64_t mu) {
     const uint64_t q = __umul64hi(a, mu);
     uint64_t r = a - q * p;
     r = r >= p ? r - p : r;
     return r >= p ? r - p : r;
+}
+
+// a * b mod p for a, b < p.  Below 2^32 the product fits 64 bits; above
+// (p < 2^40), product = hi * 2^64 + lo with hi < 2^16, and 2^64 mod p = c64.
+__device__ __forceinline__ uint64_t mulmod_b(uint64_t a, uint64_t b, uint64_t p, uint64_t mu, uint64_t c64) {
+    if (p < (1ull << 32)) return mod_b(a * b, p, mu);
+    const uint64_t lo = a * b, hi = __umul64hi(a, b);
+    return mod_b(mod_b(lo, p, mu) + mod_b(hi * c64, p, mu), p, mu);
 }
 
 constexpr uint32_t SEG_WORDS = 8192;              // 32 KiB of LDS
@@ -37,7 +47,7 @@ constexpr uint32_t SEG_BITS = SEG_WORDS * 32;     // odd numbers per segment
 // Odd primes in [3, limit]: workgroup b covers odd n = 2k+1, k in
 // [b*SEG_BITS, (b+1)*SEG_BITS).
 __global__ void __launch_bounds__(256) segment_primes_kernel(const uint32_t *__restrict__ small, uint32_t nsmall,
-                                                             uint64_t limit, uint32_t *__restrict__ out,
+                                                             uint64_t limit, uint64_t *__restrict__ out,
                                                              uint32_t *__restrict__ count, uint32_t cap) {
     __shared__ uint32_t bits[SEG_WORDS];
     for (uint32_t i = threadIdx.x; i < SEG_WORDS; i += 256) bits[i] = 0;
@@ -69,25 +79,30 @@ __global__ void __launch_bounds__(256) segment_primes_kernel(const uint32_t *__r
         while (free_bits) {
             const uint32_t b = (uint32_t)__ffs(free_bits) - 1;
             free_bits &= free_bits - 1;
-            if (base < cap) out[base] = (uint32_t)(2 * (kw + b) + 1);
+            if (base < cap) out[base] = 2 * (kw + b) + 1;
             ++base;
         }
     }
 }
 
-__global__ void mu_kernel(const uint32_t *__restrict__ p, size_t n, uint64_t *__restrict__ mu) {
+__global__ void mu_kernel(const uint64_t *__restrict__ p, size_t n, uint64_t *__restrict__ mu) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) mu[i] = ~0ull / p[i];
 }
 
-__global__ void kmod_barrett_kernel(const uint32_t *__restrict__ K, uint32_t nk, const uint32_t *__restrict__ primes,
-                                    const uint64_t *__restrict__ mu, size_t np, uint32_t *__restrict__ out) {
+// K mod p, 16 bits at a time so that r * 2^16 + digit stays below 2^64 for
+// p < 2^48.
+__global__ void kmod_barrett_kernel(const uint32_t *__restrict__ K, uint32_t nk, const uint64_t *__restrict__ primes,
+                                    const uint64_t *__restrict__ mu, size_t np, uint64_t *__restrict__ out) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= np) return;
     const uint64_t p = primes[i], u = mu[i];
     uint64_t r = 0;
-    for (int j = (int)nk - 1; j >= 0; --j) r = mod_b((r << 32) | K[j], p, u);
-    out[i] = (uint32_t)r;
+    for (int j = (int)nk - 1; j >= 0; --j) {
+        r = mod_b((r << 16) | (K[j] >> 16), p, u);
+        r = mod_b((r << 16) | (K[j] & 0xffffu), p, u);
+    }
+    out[i] = r;
 }
 
 // One workgroup per m: primes <= 2SL+1 mark the m's bitmap in LDS.
@@ -114,8 +129,8 @@ __global__ void __launch_bounds__(256) sieve_small_kernel(const uint64_t *__rest
 
 // Primes > 2SL+1 (the rest of the table) hit each interval at most once.
 __global__ void __launch_bounds__(256) sieve_large_kernel(const uint64_t *__restrict__ m, uint32_t nm, uint32_t SL,
-                                                          const uint32_t *__restrict__ primes,
-                                                          const uint32_t *__restrict__ kmod,
+                                                          const uint64_t *__restrict__ primes,
+                                                          const uint64_t *__restrict__ kmod,
                                                           const uint64_t *__restrict__ mu, size_t np,
                                                           uint32_t words, uint32_t *__restrict__ bitmap) {
     const uint32_t W = 2 * SL + 1;
@@ -123,8 +138,10 @@ __global__ void __launch_bounds__(256) sieve_large_kernel(const uint64_t *__rest
         const uint64_t p = primes[s];
         if (p <= W) continue;  // handled by sieve_small_kernel
         const uint64_t u = mu[s], kp = kmod[s];
+        uint64_t c64 = mod_b(~0ull, p, u) + 1;  // 2^64 mod p
+        if (c64 == p) c64 = 0;
         for (uint32_t j = 0; j < nm; ++j) {
-            const uint64_t r = mod_b(mod_b(m[j], p, u) * kp, p, u);
+            const uint64_t r = mulmod_b(mod_b(m[j], p, u), kp, p, u, c64);
             const uint64_t i = r <= SL ? SL - r : SL + p - r;
             if (i < W) atomicOr(&bitmap[(size_t)j * words + (i >> 5)], 1u << (i & 31));
         }
@@ -160,7 +177,7 @@ static pgap_status sieve_tables(pgap_engine *e, Family &f, uint64_t limit) {
         const double lg = std::log((double)std::max<uint64_t>(limit, 17));
         const uint32_t cap = (uint32_t)(1.26 * (double)limit / lg) + 64;
         uint32_t *d_small = nullptr, *d_count = nullptr;
-        if ((he = hipMalloc(&e->sieve_primes, (size_t)cap * 4)) != hipSuccess ||
+        if ((he = hipMalloc(&e->sieve_primes, (size_t)cap * 8)) != hipSuccess ||
             (he = hipMalloc(&e->sieve_mu, (size_t)cap * 8)) != hipSuccess ||
             (he = hipMalloc(&d_small, std::max<size_t>(small.size(), 1) * 4)) != hipSuccess ||
             (he = hipMalloc(&d_count, 4)) != hipSuccess) {
@@ -171,8 +188,9 @@ static pgap_status sieve_tables(pgap_engine *e, Family &f, uint64_t limit) {
                          hipGetErrorString(he));
             return PGAP_E_OOM;
         }
-        const uint32_t two = 2, one = 1;
-        (void)hipMemcpyAsync(e->sieve_primes, &two, 4, hipMemcpyHostToDevice, e->stream);
+        const uint64_t two = 2;
+        const uint32_t one = 1;
+        (void)hipMemcpyAsync(e->sieve_primes, &two, 8, hipMemcpyHostToDevice, e->stream);
         (void)hipMemcpyAsync(d_count, &one, 4, hipMemcpyHostToDevice, e->stream);
         (void)hipMemcpyAsync(d_small, small.data(), small.size() * 4, hipMemcpyHostToDevice, e->stream);
         const uint64_t nseg = ((limit - 1) / 2 + SEG_BITS) / SEG_BITS;
@@ -202,7 +220,7 @@ static pgap_status sieve_tables(pgap_engine *e, Family &f, uint64_t limit) {
     if (f.skmod_limit != limit) {
         if (f.d_skmod) (void)hipFree(f.d_skmod);
         f.d_skmod = nullptr;
-        if ((he = hipMalloc(&f.d_skmod, std::max<size_t>(e->sieve_nprimes, 1) * 4)) != hipSuccess) {
+        if ((he = hipMalloc(&f.d_skmod, std::max<size_t>(e->sieve_nprimes, 1) * 8)) != hipSuccess) {
             (void)hipGetLastError();
             e->set_error("pgap_sieve: hipMalloc K mod p: %s", hipGetErrorString(he));
             return PGAP_E_OOM;
@@ -222,8 +240,8 @@ extern "C" pgap_status pgap_sieve(pgap_engine *e, int family, const uint64_t *m,
     if (!e->ready) return PGAP_E_STATE;
     Family *f = e->family(family);
     if (!f || (count && (!m || !prev || !n_prev || !next || !n_next)) || max_prime < 2 ||
-        max_prime > 4000000000ull) {
-        e->set_error("pgap_sieve: bad arguments (max_prime in [2, 4e9])");
+        max_prime > 20000000000ull) {
+        e->set_error("pgap_sieve: bad arguments (max_prime in [2, 2e10])");
         return PGAP_E_ARG;
     }
     const uint32_t SL = f->SL;

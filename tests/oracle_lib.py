@@ -42,6 +42,10 @@ def load():
     lib.or_sieve_free.argtypes = [vp]
     lib.or_sieve_free.restype = None
     lib.or_sieve_m.argtypes = [vp, u64, vp, vp, vp, vp, ctypes.c_int]
+    lib.or_sieve_new_range.argtypes = [u32, u32, u32, u64, u64]
+    lib.or_sieve_new_range.restype = vp
+    lib.or_sieve_removed.argtypes = [vp, u64, u64, vp, vp, ctypes.c_int]
+    lib.or_sieve_removed.restype = ctypes.c_int
     lib.or_sieve_nprimes.argtypes = [vp]
     lib.or_sieve_nprimes.restype = sz
     lib.or_bench_verdicts.argtypes = [vp, sz, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -132,9 +136,10 @@ def scan_many(P, D, SL, ms, lists, kind=0, threads=8) -> np.ndarray:
 class Sieve:
     """Unknowns of m*K+x, x in [-SL, SL], free of primes <= max_prime."""
 
-    def __init__(self, P, D, SL, max_prime):
+    def __init__(self, P, D, SL, max_prime, above: int = 0):
+        """above > 0: keep only the primes in (above, max_prime] (for removed())."""
         self.lib = load()
-        self.h = self.lib.or_sieve_new(P, D, SL, max_prime)
+        self.h = self.lib.or_sieve_new_range(P, D, SL, above, max_prime)
         if not self.h:
             raise ValueError("bad sieve parameters")
         self.SL = SL
@@ -148,6 +153,15 @@ class Sieve:
         r = self.lib.or_sieve_m(self.h, m, prev.ctypes.data, ctypes.byref(a), nxt.ctypes.data, ctypes.byref(b), cap)
         assert r == 0
         return prev[: a.value].copy(), nxt[: b.value].copy()
+
+    def removed(self, m: int, lo: int):
+        """[(x, p)]: offsets some prime p in (lo, max_prime] removes, p the first."""
+        cap = 2 * self.SL + 1
+        xs = np.zeros(cap, dtype=np.int64)
+        ps = np.zeros(cap, dtype=np.uint64)
+        n = self.lib.or_sieve_removed(self.h, m, lo, xs.ctypes.data, ps.ctypes.data, cap)
+        assert n >= 0
+        return [(int(xs[i]), int(ps[i])) for i in range(n)]
 
     def close(self):
         if self.h:

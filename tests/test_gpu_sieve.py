@@ -33,6 +33,32 @@ def test_sieve_matches_oracle(engine, P, D, SL, limit, ms):
         assert np.array_equal(p, wp) and np.array_equal(n, wn), m
 
 
+def test_sieve_primes_above_2_32(engine):
+    # Primes past 2^32 take the 64-bit path (2^64 mod p, 16-bit K mod p
+    # steps).  Sieving to 4.5e9 must remove from the 4e9 lists exactly the
+    # offsets a prime in (4e9, 4.5e9] divides; the oracle names that prime
+    # for each offset and Python checks the division with big integers.
+    P, D, SL, ms = 907, 2190, 11000, [1, 17, 10**9 + 7]
+    lo, hi = 4_000_000_000, 4_500_000_000
+    fam = engine.family(P, D, SL)
+    low = engine.sieve(fam, ms, lo, SL)
+    high = engine.sieve(fam, ms, hi, SL)
+    sv = ol.Sieve(P, D, SL, hi, above=lo)
+    K = ol.K(P, D)
+    crossed = 0
+    for m, (lp, ln), (hp, hn) in zip(ms, low, high):
+        removed = sv.removed(m, lo)
+        for x, p in removed:
+            assert p > 2**32 or p > lo
+            assert (m * K + x) % p == 0 and ol.probab_prime(p), (m, x, p)
+        gone = {x for x, _ in removed}
+        want_p = [d for d in lp if -int(d) not in gone]
+        want_n = [d for d in ln if int(d) not in gone]
+        assert list(hp) == want_p and list(hn) == want_n, m
+        crossed += sum(1 for x, p in removed if p > 2**32)
+    assert crossed > 0
+
+
 @pytest.mark.parametrize("flag,md5", [(None, "15a5cbff7301262caf047028c05f0525"),
                                       ("--rle", "2ed7b0b9621dab602f204865360a3c56"),
                                       ("--bitcompressed", "6edd5f50c4f588890a168461a7240c47")])
