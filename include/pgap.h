@@ -98,7 +98,13 @@ typedef struct {
     uint32_t lanes_per_number;
     uint32_t launches;
     uint64_t tests;
+    uint32_t graph;             /* 1 when the launch replayed a captured hipGraph */
+    uint32_t reserved;
 } pgap_launch_info;
+/* Timing and shape of the last batch launch.  A batch call repeated with
+ * exactly the same arguments (variant, buffers, count, bases) is captured
+ * into a hipGraph on its second occurrence and replayed from then on, so a
+ * steady-state loop costs one graph launch per call. */
 pgap_status pgap_last_launch(const pgap_engine *e, pgap_launch_info *info);
 
 /* ---- streaming scan (serving) ----------------------------------------- */

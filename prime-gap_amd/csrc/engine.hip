@@ -136,43 +136,103 @@ __global__ void select_kernel(const uint8_t *__restrict__ verdict, uint32_t coun
     if (t < count && verdict[t]) sel[atomicAdd(nsel, 1u)] = t;
 }
 
-pgap_status pgap_engine::launch_batch(const Choice &c, const uint32_t *d_n, uint32_t limbs, size_t count,
-                                      const Bases &bases, uint8_t *d_out) {
+// A counter reset as a kernel, not a memset node, so the sequence can sit in
+// a captured graph.
+__global__ void zero_kernel(uint32_t *__restrict__ p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
+// Several strong-PRP bases: the first on every row, each later one only on
+// the rows that passed so far (selected on the device), so lanes that
+// already failed do not ride along through the later bases.
+hipError_t pgap_engine::enqueue_batch(const Choice &c, const uint32_t *d_n, uint32_t limbs, size_t count,
+                                      const Bases &bases, uint8_t *d_out, uint32_t *launches) {
     const Variant *v = c.v;
-    hipError_t e = hipEventRecord(ev0, stream);
-    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
-    // Grid dimensions are 32-bit: split very large batches.
-    const size_t chunk = (size_t)1 << 30;
-    const int cap = c.cap;
-    uint32_t launches = 0;
-    // Several strong-PRP bases: the first on every row, each later one only
-    // on the rows that passed so far (selected on the device), so lanes that
-    // already failed do not ride along through the later bases.
+    const size_t chunk = (size_t)1 << 30;  // grid dimensions are 32-bit: split very large batches
     const bool staged = v->kind == KIND_SPRP && bases.n > 1 && count <= chunk;
     Bases first = bases;
     if (staged) first.n = 1;
+    hipError_t e;
     for (size_t off = 0; off < count; off += chunk) {
         const uint32_t n = (uint32_t)std::min(chunk, count - off);
-        e = v->launch(d_n + off * limbs, limbs, n, nullptr, nullptr, first, d_out + off, cap, stream);
-        if (e != hipSuccess) return hip_fail(e, "kernel launch");
-        ++launches;
+        if ((e = v->launch(d_n + off * limbs, limbs, n, nullptr, nullptr, first, d_out + off, c.cap, stream)) !=
+            hipSuccess)
+            return e;
+        ++*launches;
     }
-    if (staged) {
+    if (!staged) return hipSuccess;
+    uint32_t *sel = (uint32_t *)d_sel.p, *nsel = (uint32_t *)d_nsel.p;
+    for (uint32_t k = 1; k < bases.n; ++k) {
+        Bases one{};
+        one.n = 1;
+        one.b[0] = bases.b[k];
+        hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, nsel);
+        hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0, stream, d_out,
+                           (uint32_t)count, sel, nsel);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = v->launch(d_n, limbs, (uint32_t)count, nsel, sel, one, d_out, c.cap, stream)) != hipSuccess)
+            return e;
+        ++*launches;
+    }
+    return hipSuccess;
+}
+
+pgap_status pgap_engine::launch_batch(const Choice &c, const uint32_t *d_n, uint32_t limbs, size_t count,
+                                      const Bases &bases, uint8_t *d_out) {
+    const Variant *v = c.v;
+    if (v->kind == KIND_SPRP && bases.n > 1) {
         pgap_status st;
         if ((st = ensure(d_sel, count * 4, "survivor index")) || (st = ensure(d_nsel, 64, "survivor count")))
             return st;
-        uint32_t *sel = (uint32_t *)d_sel.p, *nsel = (uint32_t *)d_nsel.p;
-        for (uint32_t k = 1; k < bases.n; ++k) {
-            Bases one{};
-            one.n = 1;
-            one.b[0] = bases.b[k];
-            if ((e = hipMemsetAsync(nsel, 0, 4, stream)) != hipSuccess) return hip_fail(e, "survivor count");
-            hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0, stream, d_out,
-                               (uint32_t)count, sel, nsel);
-            e = v->launch(d_n, limbs, (uint32_t)count, nsel, sel, one, d_out, cap, stream);
-            if (e != hipSuccess) return hip_fail(e, "kernel launch");
-            ++launches;
+    }
+    BatchGraph key;
+    key.v = v;
+    key.d_n = d_n;
+    key.limbs = limbs;
+    key.count = count;
+    key.bases = bases;
+    key.d_out = d_out;
+    key.cap = c.cap;
+    key.sel = d_sel.p;
+    key.nsel = d_nsel.p;
+    const bool repeat = graph.same(key);
+    if (!repeat && graph.exec) {
+        (void)hipGraphExecDestroy(graph.exec);
+        graph.exec = nullptr;
+    }
+    hipError_t e = hipEventRecord(ev0, stream);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    uint32_t launches = 0;
+    bool replayed = false;
+    if (repeat) {
+        // the same launch as last time: capture it once, then replay
+        if (!graph.exec) {
+            hipGraph_t g = nullptr;
+            if ((e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+                return hip_fail(e, "graph capture");
+            const hipError_t le = enqueue_batch(c, d_n, limbs, count, bases, d_out, &launches);
+            e = hipStreamEndCapture(stream, &g);
+            if (le != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return hip_fail(le, "kernel launch (capture)");
+            }
+            if (e != hipSuccess) return hip_fail(e, "graph capture end");
+            e = hipGraphInstantiate(&graph.exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                graph.exec = nullptr;
+                return hip_fail(e, "graph instantiate");
+            }
+        } else {
+            launches = graph_launches;
         }
+        if ((e = hipGraphLaunch(graph.exec, stream)) != hipSuccess) return hip_fail(e, "graph launch");
+        replayed = true;
+        graph_launches = launches;
+    } else {
+        if ((e = enqueue_batch(c, d_n, limbs, count, bases, d_out, &launches)) != hipSuccess)
+            return hip_fail(e, "kernel launch");
+        graph = key;
     }
     e = hipEventRecord(ev1, stream);
     if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
@@ -181,6 +241,7 @@ pgap_status pgap_engine::launch_batch(const Choice &c, const uint32_t *d_n, uint
     last.lanes_per_number = (uint32_t)v->lanes;
     last.launches = launches;
     last.tests = count;
+    last.graph = replayed ? 1u : 0u;
     last.kernel_ms = -1.0;
     timing_pending = true;
     return PGAP_OK;
@@ -291,6 +352,8 @@ void pgap_destroy(pgap_engine *e) {
         e->scan_shutdown();
         e->sieve_shutdown();
         e->families_shutdown();
+        if (e->graph.exec) (void)hipGraphExecDestroy(e->graph.exec);
+        e->graph.exec = nullptr;
         for (DevBuf *b : {&e->d_n, &e->d_out, &e->d_small, &e->d_m, &e->d_x, &e->d_keep, &e->d_idx0, &e->d_idx1,
                           &e->d_sel, &e->d_nsel}) {
             if (b->p) (void)hipFree(b->p);

@@ -35,6 +35,28 @@ struct Choice {
     int cap = 0;
 };
 
+// The last batch launch's arguments and, once the same launch has come
+// twice in a row, its captured graph.
+struct BatchGraph {
+    const Variant *v = nullptr;
+    const uint32_t *d_n = nullptr;
+    uint32_t limbs = 0;
+    size_t count = 0;
+    Bases bases{};
+    uint8_t *d_out = nullptr;
+    int cap = 0;
+    void *sel = nullptr, *nsel = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool same(const BatchGraph &o) const {
+        if (v != o.v || d_n != o.d_n || limbs != o.limbs || count != o.count || d_out != o.d_out || cap != o.cap ||
+            sel != o.sel || nsel != o.nsel || bases.n != o.bases.n)
+            return false;
+        for (uint32_t i = 0; i < bases.n; ++i)
+            if (bases.b[i] != o.bases.b[i]) return false;
+        return true;
+    }
+};
+
 // K = P#/d with what the device needs to build and screen m*K + x.
 struct Family {
     uint32_t P = 0, D = 0, SL = 0;
@@ -75,6 +97,10 @@ struct pgap_engine {
 
     pgap::DevBuf d_n, d_out, d_small, d_m, d_x, d_keep, d_idx0, d_idx1;
     pgap::DevBuf d_sel, d_nsel;  // survivors between strong-PRP bases
+    pgap::BatchGraph graph;       // steady-state replay of a repeated batch launch
+    uint32_t graph_launches = 0;  // kernel launches inside the captured graph
+    hipError_t enqueue_batch(const pgap::Choice &c, const uint32_t *d_n, uint32_t limbs, size_t count,
+                             const pgap::Bases &bases, uint8_t *d_out, uint32_t *launches);
     std::vector<pgap::Family> families;  // appended only by pgap_family (client thread)
     pgap::Scanner *scanner = nullptr;  // owned; released by scan_shutdown()
 

@@ -79,6 +79,8 @@ class LaunchInfo(ctypes.Structure):
         ("lanes_per_number", ctypes.c_uint32),
         ("launches", ctypes.c_uint32),
         ("tests", ctypes.c_uint64),
+        ("graph", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
