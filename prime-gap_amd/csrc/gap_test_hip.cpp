@@ -22,8 +22,20 @@
 // --results FILE (tab-separated P D m next_p prev_p merit, the columns of the
 // reference's result table, schema.sql:142-154), --dump-unknowns (parse and
 // print the lists, no GPU).
+//
+// --db FILE records every finished m in an SQLite file the way gap_test.py
+// does through gap_test_stats.save (gap_test_stats.py:217-238): a `result`
+// row (next_p 0 for a side-skipped m) and the m_stats columns next_p, prev_p,
+// merit, prp_next, prp_prev, test_time (a row is added when gap_stats did not
+// create one).  m's whose `result` row already has next_p >= 0 are not tested
+// again (load_existing, gap_test_stats.py:145-150, and the resume rule of
+// gap_test.py:341-349).  test_time is the m's time in flight in the scan.
+// --prp-top-percent X (with --db) tests only the m's whose m_stats
+// prob_record is at least the X% threshold (determine_test_threshold,
+// gap_test_stats.py:456-486); partial rows are tested whatever it says.
 #include <getopt.h>
 
+#include <algorithm>
 #include <chrono>
 #include <clocale>
 #include <cmath>
@@ -31,11 +43,13 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
 #include "../../include/pgap.h"
 #include "hostbn.h"
+#include "resultdb.h"
 #include "unknowns.h"
 
 using namespace pgap;
@@ -49,7 +63,8 @@ struct Args {
     int verbose = 2;
     uint32_t depth = 256, confirm = 2;
     size_t hbm_budget = 0;
-    std::string results;
+    std::string results, db;
+    int top_percent = 0;
     bool dump = false;
     bool side_skip = false;
     bool valid = true;
@@ -66,6 +81,8 @@ void usage(const char *name) {
     printf("  --hbm-budget-mb <n>  device memory the scan may keep resident\n");
     printf("  --side-skip       gap_test_gpu order: prev side first, skip next if prev merit < min/2-2\n");
     printf("  --results <file>  append result rows (P D m next_p prev_p merit, tab separated)\n");
+    printf("  --db <file>       record result / m_stats rows in an SQLite file; skip m's it holds\n");
+    printf("  --prp-top-percent <x>  only test the top x%% of m's by m_stats.prob_record (needs --db)\n");
     printf("  --dump-unknowns   print the parsed unknown lists and exit (no GPU)\n");
     printf("  -q, --quiet       less output (twice for less)\n  -h, --help\n");
 }
@@ -86,7 +103,8 @@ Args parse_args(int argc, char **argv) {
         {"max-prime", required_argument, 0, 5},          {"depth", required_argument, 0, 20},
         {"confirm", required_argument, 0, 21},           {"hbm-budget-mb", required_argument, 0, 22},
         {"results", required_argument, 0, 23},           {"dump-unknowns", no_argument, 0, 24},
-        {"side-skip", no_argument, 0, 25},
+        {"side-skip", no_argument, 0, 25},               {"db", required_argument, 0, 26},
+        {"prp-top-percent", required_argument, 0, 27},
         {"method1", no_argument, 0, 8},                  {"quiet", no_argument, 0, 'q'},
         {"help", no_argument, 0, 'h'},                   {0, 0, 0, 0}};
     Args a;
@@ -111,6 +129,8 @@ Args parse_args(int argc, char **argv) {
             case 23: a.results = optarg; break;
             case 24: a.dump = true; break;
             case 25: a.side_skip = true; break;
+            case 26: a.db = optarg; break;
+            case 27: a.top_percent = atoi(optarg); break;
             case 'u':
                 if (!parse_unknown_filename(optarg, a.u, err)) {
                     fprintf(stderr, "%s\n", err.c_str());
@@ -130,6 +150,10 @@ Args parse_args(int argc, char **argv) {
     if (a.u.d == 0) { printf("d must be greater than 0: %u\n", a.u.d); a.valid = false; }
     if (!is_prime_small(a.u.p)) { printf("p# not prime (p=%u)\n", a.u.p); a.valid = false; }
     if (a.u.sieve_length == 0) { printf("Must set sieve-length\n"); a.valid = false; }
+    if (a.top_percent && (a.top_percent < 1 || a.top_percent > 100 || a.db.empty())) {
+        printf("--prp-top-percent takes 1..100 and needs --db (m_stats rows from gap_stats)\n");
+        a.valid = false;
+    }
     if (a.mskip && (a.mskip < a.u.mstart || a.mskip >= a.u.mstart + a.u.minc)) {
         printf("mskip must lie in [mstart, mstart + minc)\n");
         a.valid = false;
@@ -256,6 +280,56 @@ int main(int argc, char **argv) {
             ++valid_ms;
             last_mi = mi;
         }
+
+    // m's this run leaves out: finished rows of --db, and with
+    // --prp-top-percent those below the probability threshold
+    ResultDb db;
+    std::set<uint64_t> skip_m, partial;
+    const uint64_t m_hi = a.u.mstart + a.u.minc - 1;
+    if (!a.db.empty()) {
+        if (!db.open(a.db) || !db.existing(a.u.p, a.u.d, a.u.mstart, m_hi, skip_m, partial)) {
+            fprintf(stderr, "database '%s': %s\n", a.db.c_str(), db.error().c_str());
+            return 1;
+        }
+        if (a.verbose >= 0)
+            printf("Found %lu results in '%s'\n", (unsigned long)(skip_m.size() + partial.size()), a.db.c_str());
+    }
+    if (a.top_percent > 0) {
+        // determine_test_threshold (gap_test_stats.py:456-486)
+        std::vector<std::pair<uint64_t, float>> probs;
+        if (!db.probs(a.u.p, a.u.d, a.u.mstart, m_hi, probs)) {
+            fprintf(stderr, "database '%s': %s (run gap_stats first)\n", a.db.c_str(), db.error().c_str());
+            return 1;
+        }
+        if (probs.size() != valid_ms) {
+            fprintf(stderr, "m_stats holds %lu rows for this range, expected %lu (run gap_stats first)\n",
+                    (unsigned long)probs.size(), (unsigned long)valid_ms);
+            return 1;
+        }
+        if (a.top_percent < 100) {
+            std::vector<float> best;
+            for (auto &pr : probs) best.push_back(pr.second);
+            std::sort(best.begin(), best.end(), std::greater<float>());
+            // Python's round(): halves go to the even neighbour
+            size_t at = (size_t)std::nearbyint(best.size() * (double)a.top_percent / 100.0);
+            const float thr = best[std::min(at, best.size() - 1)];
+            size_t n_sel = 0;
+            for (auto &pr : probs) {
+                if (pr.second >= thr)
+                    ++n_sel;
+                else if (!partial.count(pr.first))  // partial rows run whatever their probability
+                    skip_m.insert(pr.first);
+            }
+            if (a.verbose >= 0) printf("Testing %lu m where prob(record) >= %.3g\n", (unsigned long)n_sel, thr);
+        }
+    }
+    // the last m this run tests: its row closes the stats
+    uint64_t stats_last_mi = last_mi;
+    for (uint64_t mi = last_mi + 1; mi-- > 0;)
+        if (gcd64(a.u.mstart + mi, a.u.d) == 1 && !skip_m.count(a.u.mstart + mi)) {
+            stats_last_mi = mi;
+            break;
+        }
     const float min_merit = a.min_merit;
     if (a.verbose >= 1) {
         const float m_log = (float)log((double)a.u.mstart);
@@ -313,9 +387,14 @@ int main(int argc, char **argv) {
                 if (results)
                     fprintf(results, "%u\t%u\t%ld\t%d\t%d\t%.4f\n", a.u.p, a.u.d, m, next_p, prev_p, merit);
             }
+            if (!a.db.empty() && !db.save(a.u.p, a.u.d, r.m, next_p, prev_p, gap / (K_log + log((double)m)),
+                                          r.n_tests, r.p_tests, r.t_done - r.t_submit)) {
+                fprintf(stderr, "database '%s': %s\n", a.db.c_str(), db.error().c_str());
+                status = 1;
+            }
             stats.fallback_prev += (r.flags & PGAP_ROW_PREV_FALLBACK) ? 1 : 0;
             stats.fallback_next += (r.flags & PGAP_ROW_NEXT_FALLBACK) ? 1 : 0;
-            const bool is_last = (r.m - a.u.mstart) == last_mi;
+            const bool is_last = (r.m - a.u.mstart) == stats_last_mi;
             stats.process(a.verbose, a.u.sieve_length, m, is_last, it->second.ul, it->second.uu, prev_p, next_p,
                           r.p_tests, r.n_tests, merit);
             done.erase(d);
@@ -344,7 +423,7 @@ int main(int argc, char **argv) {
             status = 1;
             break;
         }
-        if (m < a.mskip) continue;
+        if (m < a.mskip || skip_m.count(m)) continue;
         uint64_t mp = 0;
         if (!parse_unknown_line(line, format, format == 2 ? &bl : nullptr, mp, prev, next, err) || mp != m) {
             fprintf(stderr, "bad line for m=%lu: %s\n", (unsigned long)m, mp != m ? "m mismatch" : err.c_str());
@@ -383,6 +462,7 @@ int main(int argc, char **argv) {
         if (!drain(true)) status = 1;
     if (status == 0) flush(true);
     if (results) fclose(results);
+    db.close();
     pgap_destroy(e);
     return status;
 }

@@ -118,11 +118,9 @@ __device__ __forceinline__ bool equal(const uint32_t (&a)[L], const uint32_t (&b
 // a <- a^2 * R^{-1} * 2^s mod N (s in {0,1}), lazy: a < 4N in, a < 4N out.
 // The 2^s factor is folded into the high (result) columns, so the doubling
 // step of a base-2 exponentiation costs two VALU slots per digit of output.
-//
-// Each column is summed in three independent chains -- the cross products,
-// and the m*N products in two halves -- issued round-robin and merged once
-// per column, so consecutive multiply-adds do not wait on each other (exact
-// integer sums: the result does not depend on the grouping).
+// One dependent chain per column: the narrow variants keep two or more
+// wavefronts per SIMD, which hide the multiply-add latency (splitting the
+// column into independent chains measured 1% slower at 40 digits).
 template <int L>
 __device__ __forceinline__ void sqr_redc(uint32_t (&a)[L], const uint32_t (&n)[L], uint32_t np,
                                          uint32_t s) {
@@ -132,29 +130,24 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&a)[L], const uint32_t (&n)[L
 #pragma unroll
     for (int k = 0; k < 2 * L - 1; ++k) {
         const int ilo = k - (L - 1) > 0 ? k - (L - 1) : 0;
-        const int xe = (k + 1) / 2;        // cross products a_i a_{k-i}, i in [ilo, xe)
-        const int ye = k < L ? k : L;      // m_i n_{k-i}, i in [ilo, ye) (m_k joins below)
-        const int ym = (ilo + ye + 1) / 2; // split point of the m*N chain
-        const int nx = xe - ilo, n0 = ym - ilo, n1 = ye - ym;
-        const int rounds = nx > n0 ? (nx > n1 ? nx : n1) : (n0 > n1 ? n0 : n1);
-        uint64_t X = 0, Y0 = k < L ? T : 0, Y1 = 0;
+        // cross products a_i * a_{k-i}, i < k-i
+        uint64_t X = 0;
 #pragma unroll
-        for (int r = 0; r < rounds; ++r) {
-            if (r < nx) X = mad(a[ilo + r], a[k - ilo - r], X);
-            if (r < n0) Y0 = mad(m[ilo + r], n[k - ilo - r], Y0);
-            if (r < n1) Y1 = mad(m[ym + r], n[k - ym - r], Y1);
-        }
-        // opaque to the optimiser: otherwise reassociation folds the three
-        // sums back into one dependent chain
-        asm("" : "+v"(X), "+v"(Y0), "+v"(Y1));
-        uint64_t Z = (X << 1) + Y0;
-        Z += Y1;
-        if ((k & 1) == 0) Z = mad(a[k >> 1], a[k >> 1], Z);
+        for (int i = ilo; 2 * i < k; ++i) X = mad(a[i], a[k - i], X);
         if (k < L) {
-            m[k] = ((uint32_t)Z * np) & MASK;
-            T = mad(m[k], n[0], Z);
+            T = (X << 1) + T;
+            if ((k & 1) == 0) T = mad(a[k >> 1], a[k >> 1], T);
+#pragma unroll
+            for (int i = ilo; i < k; ++i) T = mad(m[i], n[k - i], T);
+            m[k] = ((uint32_t)T * np) & MASK;
+            T = mad(m[k], n[0], T);
             T >>= 28;
         } else {
+            uint64_t Z = 0;
+            if ((k & 1) == 0) Z = mad(a[k >> 1], a[k >> 1], Z);
+#pragma unroll
+            for (int i = ilo; i < L; ++i) Z = mad(m[i], n[k - i], Z);
+            Z = (X << 1) + Z;
             if (k == L) T <<= s;
             T = (Z << s) + T;
             // digit k-L of the result; a[k-L] and m[k-L] are dead here

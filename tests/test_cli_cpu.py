@@ -96,3 +96,81 @@ def test_malformed_and_truncated_lines_fail(formats, tmp_path):
     # missing file
     r = run("-u", str(tmp_path / "x" / FN), "--dump-unknowns")
     assert r.returncode != 0
+
+
+def test_db_tables_and_resume(formats, tmp_path):
+    # --db: the result / m_stats tables are created when absent, and an m whose
+    # result row has next_p >= 0 is not read again (gap_test.py:341-349: 0 is a
+    # side-skipped m, -1 a partial row that is tested again); rows of another
+    # D are not this range's.
+    import sqlite3
+
+    db = tmp_path / "gaps.db"
+    r = run("-u", formats["plain"], "--dump-unknowns", "-qqq", "--db", str(db))
+    assert r.returncode == 0 and hashlib.md5(r.stdout).hexdigest() == PLAIN_MD5
+    con = sqlite3.connect(db)
+    cols = {t: [c[1] for c in con.execute(f"PRAGMA table_info({t})")] for t in ("result", "m_stats")}
+    assert cols["result"] == ["P", "D", "m", "next_p", "prev_p", "merit"]
+    assert {"next_p", "prev_p", "merit", "prp_next", "prp_prev", "test_time"} <= set(cols["m_stats"])
+    con.executemany("INSERT INTO result VALUES(?, ?, ?, ?, ?, ?)",
+                    [(907, 2190, 7, 10, 20, 1.0), (907, 2190, 11, 0, 30, 0.5),
+                     (907, 2190, 13, -1, 40, 0.0), (907, 2310, 17, 8, 8, 0.1)])
+    con.commit()
+    con.close()
+    r = run("-u", formats["plain"], "--dump-unknowns", "-q", "--db", str(db))
+    assert r.returncode == 0, r.stderr
+    out = r.stdout.decode()
+    assert "Found 3 results" in out
+    ms = {int(l.split(" ")[0]) for l in out.splitlines() if l[:1].isdigit() and " : " in l}
+    assert 7 not in ms and 11 not in ms and {1, 13, 17, 199} <= ms
+    assert len(ms) == 53 - 2
+
+
+def test_db_unwritable_path_fails(formats, tmp_path):
+    r = run("-u", formats["plain"], "--dump-unknowns", "-qqq", "--db", str(tmp_path / "no" / "dir" / "x.db"))
+    assert r.returncode != 0 and b"database" in r.stderr
+
+
+def test_db_prp_top_percent(formats, tmp_path):
+    # determine_test_threshold (gap_test_stats.py:456-486): the threshold is
+    # the prob_record at index round(n * percent / 100) of the descending
+    # list; an m at or above it is tested, and so is a partial row.
+    import sqlite3
+
+    import numpy as np
+
+    lines = open(formats["plain"], "rb").read().splitlines()
+    ms = [int(l.split(b" ")[0]) for l in lines]
+    db = tmp_path / "gaps.db"
+    con = sqlite3.connect(db)
+    con.execute("CREATE TABLE m_stats (P INTEGER, D INTEGER, m INTEGER, next_p INTEGER DEFAULT 0, "
+                "prev_p INTEGER DEFAULT 0, merit REAL DEFAULT 0, prob_record REAL, prp_next INTEGER DEFAULT 0, "
+                "prp_prev INTEGER DEFAULT 0, test_time REAL DEFAULT 0, PRIMARY KEY(P, D, m))")
+    probs = {m: ((m * 7919) % 101) / 1e4 for m in ms}
+    con.executemany("INSERT INTO m_stats(P, D, m, prob_record) VALUES(907, 2190, ?, ?)", probs.items())
+    con.execute("CREATE TABLE result (P INTEGER, D INTEGER, m INTEGER, next_p INTEGER, prev_p INTEGER, "
+                "merit REAL, PRIMARY KEY(P, D, m))")
+    low = min(ms, key=lambda m: probs[m])
+    con.execute("INSERT INTO result VALUES(907, 2190, ?, -1, 50, 0)", (low,))
+    con.commit()
+    con.close()
+    for percent in (5, 10, 37, 100):
+        r = run("-u", formats["plain"], "--dump-unknowns", "-q", "--db", str(db), "--prp-top-percent", str(percent))
+        assert r.returncode == 0, r.stderr
+        got = {int(l.split(" ")[0]) for l in r.stdout.decode().splitlines() if l[:1].isdigit() and " : " in l}
+        f = {m: np.float32(v) for m, v in probs.items()}
+        if percent == 100:
+            want = set(ms)
+        else:
+            best = sorted(f.values(), reverse=True)
+            thr = best[round(len(best) * percent / 100)]
+            want = {m for m in ms if f[m] >= thr} | {low}
+        assert got == want, percent
+
+
+def test_db_prp_top_percent_needs_stats(formats, tmp_path):
+    r = run("-u", formats["plain"], "--dump-unknowns", "--prp-top-percent", "10")
+    assert r.returncode != 0
+    # no m_stats rows from gap_stats in a fresh database
+    r = run("-u", formats["plain"], "--dump-unknowns", "--db", str(tmp_path / "x.db"), "--prp-top-percent", "10")
+    assert r.returncode != 0 and b"gap_stats" in r.stderr

@@ -115,3 +115,90 @@ def test_cli_side_skip_rows_and_count():
     assert got_rows == want_rows
     assert 0 < skips
     assert re.findall(r"only prev_prime (\d+) ", out)[-1] == str(skips)
+
+
+def _fixture_rows():
+    return {uf.parse_line(ln)[0]: uf.parse_line(ln) for ln in open(FIXTURE, "rb").read().splitlines(keepends=True)}
+
+
+def test_cli_db_rows_m_stats_and_resume(tmp_path):
+    # gap_test_stats.save (gap_test_stats.py:217-238): REPLACE INTO result,
+    # m_stats next_p / prev_p / merit set and prp_next / prp_prev / test_time
+    # added to; load_existing (gap_test_stats.py:145-150): a second run over
+    # the range tests only the m's without a finished row.
+    import sqlite3
+
+    db = tmp_path / "gaps.db"
+    lines = _fixture_rows()
+    K_log = ol.K_log(907, 2190)
+    con = sqlite3.connect(db)
+    # m_stats as gap_stats leaves it: one row per m, with columns of its own
+    con.execute("CREATE TABLE m_stats (P INTEGER, D INTEGER, m INTEGER, next_p INTEGER DEFAULT 0, "
+                "prev_p INTEGER DEFAULT 0, merit REAL DEFAULT 0, prob_record REAL, prp_next INTEGER DEFAULT 0, "
+                "prp_prev INTEGER DEFAULT 0, test_time REAL DEFAULT 0, PRIMARY KEY(P, D, m))")
+    con.executemany("INSERT INTO m_stats(P, D, m, prob_record, prp_next, prp_prev, test_time) "
+                    "VALUES(907, 2190, ?, ?, 1, 2, 0.5)", [(m, m / 1000) for m in lines])
+    con.commit()
+    con.close()
+
+    r = subprocess.run([CLI, "-u", FIXTURE, "-qq", "--mskip", "150", "--db", str(db)], capture_output=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr.decode()
+    con = sqlite3.connect(db)
+    res = {row[0]: row[1:] for row in con.execute("SELECT m, next_p, prev_p, merit FROM result WHERE P = 907 AND D = 2190")}
+    assert sorted(res) == [m for m in sorted(lines) if m >= 150]
+    for m, (next_p, prev_p, merit) in res.items():
+        _, p, n = lines[m]
+        w = ol.test_interval(907, 2190, m, 11000, p, n, kind=0)
+        assert (prev_p, next_p) == (w["prev_p"], w["next_p"]), m
+        assert abs(merit - round((prev_p + next_p) / (K_log + math.log(m)), 4)) < 1.5e-4, m
+        st = con.execute("SELECT next_p, prev_p, merit, prp_next, prp_prev, test_time, prob_record FROM m_stats "
+                         "WHERE P = 907 AND D = 2190 AND m = ?", (m,)).fetchone()
+        assert st[:3] == (next_p, prev_p, merit)
+        assert (st[3], st[4]) == (1 + w["n_tests"], 2 + w["p_tests"]), m
+        assert st[5] > 0.5 and st[6] == m / 1000
+    untouched = con.execute("SELECT COUNT(*) FROM m_stats WHERE m < 150 AND prp_next = 1 AND next_p = 0").fetchone()
+    assert untouched[0] == sum(1 for m in lines if m < 150)
+    con.close()
+
+    # resume: the whole range again -- only m < 150 are tested now
+    r = subprocess.run([CLI, "-u", FIXTURE, "-q", "--db", str(db)], capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stderr.decode()
+    assert f"Found {len(res)} results" in r.stdout.decode()
+    con = sqlite3.connect(db)
+    assert con.execute("SELECT COUNT(*) FROM result").fetchone()[0] == len(lines)
+    # rows of the first run were not tested twice: their prp counts stand
+    for m in res:
+        _, p, n = lines[m]
+        w = ol.test_interval(907, 2190, m, 11000, p, n, kind=0)
+        st = con.execute("SELECT prp_next, prp_prev FROM m_stats WHERE m = ?", (m,)).fetchone()
+        assert st == (1 + w["n_tests"], 2 + w["p_tests"])
+    con.close()
+
+
+def test_cli_db_side_skip_rows(tmp_path):
+    # a side-skipped m is saved with next_p = 0 and the prev side's merit
+    # (gap_test.py:164-165, gap_test_stats.py:574-577); its m_stats row is
+    # added when gap_stats made none.
+    import sqlite3
+
+    db = tmp_path / "gaps.db"
+    min_merit = 6.0
+    r = subprocess.run([CLI, "-u", FIXTURE, "-qq", "--min-merit", str(min_merit), "--side-skip", "--db", str(db)],
+                       capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stderr.decode()
+    K_log = ol.K_log(907, 2190)
+    thr = np.float32(np.float32(min_merit) / np.float32(2) - np.float32(2))
+    con = sqlite3.connect(db)
+    skipped = 0
+    for m, (_, p, n) in _fixture_rows().items():
+        w = ol.test_interval(907, 2190, m, 11000, p, n, kind=0)
+        row = con.execute("SELECT next_p, prev_p FROM result WHERE m = ?", (m,)).fetchone()
+        st = con.execute("SELECT next_p, prev_p, prp_next FROM m_stats WHERE m = ?", (m,)).fetchone()
+        if np.float32(w["prev_p"] / (K_log + math.log(m))) < thr:
+            skipped += 1
+            assert row == (0, w["prev_p"]) and st == (0, w["prev_p"], 0), m
+        else:
+            assert row == (w["next_p"], w["prev_p"]) and st == (w["next_p"], w["prev_p"], w["n_tests"]), m
+    assert skipped > 0
+    con.close()
