@@ -24,20 +24,25 @@ struct Buckets {
 };
 using MixedBuckets = Buckets<12, 26, 40, 56, 78>;
 
+// LDS park of the widest bucket (the last), one column per thread.
+constexpr size_t kMixedParkBytes =
+    (size_t)MixedBuckets::digits[MixedBuckets::n - 1] * r28::PARK_STRIDE * sizeof(uint32_t);
+
 template <int KIND, int L>
-__device__ __forceinline__ uint32_t test_one(const uint32_t *nw, uint32_t limbs, const Bases &bases) {
-    if (KIND == KIND_FERMAT2) return r28::fermat2<L>(nw, limbs);
+__device__ __forceinline__ uint32_t test_one(const uint32_t *nw, uint32_t limbs, const Bases &bases,
+                                             uint32_t *park) {
+    if (KIND == KIND_FERMAT2) return r28::fermat2<L>(nw, limbs, park);
     uint32_t v = 1;
-    for (uint32_t k = 0; k < bases.n && v; ++k) v = r28::sprp<L>(nw, limbs, bases.b[k]);
+    for (uint32_t k = 0; k < bases.n && v; ++k) v = r28::sprp<L>(nw, limbs, bases.b[k], park);
     return v;
 }
 
 template <int KIND, int... Ls>
 __device__ __forceinline__ uint32_t dispatch(int bucket, const uint32_t *nw, uint32_t limbs, const Bases &bases,
-                                             Buckets<Ls...>) {
+                                             uint32_t *park, Buckets<Ls...>) {
     uint32_t v = 0;
     int i = 0;
-    ((bucket == i++ ? (void)(v = test_one<KIND, Ls>(nw, limbs, bases)) : (void)0), ...);
+    ((bucket == i++ ? (void)(v = test_one<KIND, Ls>(nw, limbs, bases, park)) : (void)0), ...);
     return v;
 }
 
@@ -49,6 +54,8 @@ __global__ void __launch_bounds__(256) mixed_kernel(const uint32_t *__restrict__
                                                     const uint8_t *__restrict__ tile_bucket, uint32_t ntiles,
                                                     uint32_t *__restrict__ next_tile, Bases bases,
                                                     uint8_t *__restrict__ out) {
+    extern __shared__ uint32_t mixed_park[];
+    uint32_t *park = mixed_park + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63;
     for (;;) {
         uint32_t t = 0;
@@ -58,7 +65,7 @@ __global__ void __launch_bounds__(256) mixed_kernel(const uint32_t *__restrict__
         const int bucket = tile_bucket[t];
         const uint32_t item = order[(size_t)t * 64 + lane];
         if (item != 0xffffffffu) {
-            const uint32_t v = dispatch<KIND>(bucket, words + offset[item], limbs[item], bases, MixedBuckets{});
+            const uint32_t v = dispatch<KIND>(bucket, words + offset[item], limbs[item], bases, park, MixedBuckets{});
             out[item] = (uint8_t)v;
         }
     }
@@ -83,11 +90,14 @@ hipError_t launch_mixed_sprp(const MixedLaunch &a, int device, hipStream_t strea
 template <int KIND>
 hipError_t launch_mixed(const MixedLaunch &a, int device, hipStream_t stream) {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mixed_kernel<KIND>, 256, 0) != hipSuccess || per_cu <= 0)
+    (void)hipFuncSetAttribute((const void *)mixed_kernel<KIND>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kMixedParkBytes);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mixed_kernel<KIND>, 256, kMixedParkBytes) != hipSuccess ||
+        per_cu <= 0)
         per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 256;
     const uint32_t grid = (uint32_t)std::min<size_t>((size_t)per_cu * cus, ((size_t)a.ntiles + 3) / 4);
-    hipLaunchKernelGGL(mixed_kernel<KIND>, dim3(std::max(grid, 1u)), dim3(256), 0, stream, a.words, a.offset, a.limbs,
+    hipLaunchKernelGGL(mixed_kernel<KIND>, dim3(std::max(grid, 1u)), dim3(256), kMixedParkBytes, stream, a.words, a.offset, a.limbs,
                        a.order, a.tile_bucket, a.ntiles, a.next_tile, a.bases, a.out);
     return hipGetLastError();
 }

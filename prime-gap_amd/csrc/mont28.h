@@ -115,6 +115,52 @@ __device__ __forceinline__ bool equal(const uint32_t (&a)[L], const uint32_t (&b
     return x == 0;
 }
 
+// Classifies x * R^{-1} mod N for a Montgomery-form x < R (every lazy bound
+// used here qualifies: x < 4N with N < R/16, x < 128N with N < R/2^14): 1 if
+// it is 1, 2 if it is N - 1, 0 otherwise.  This is the Montgomery reduction
+// of x itself, (x + mN)/R, which is congruent to x R^{-1} and lies in [0, N]
+// because x < R and m < R, so the two residues are recognised by their
+// digits alone: no Mont(1) or Mont(-1) array is held or rebuilt, and nothing
+// beyond x, N and one working copy is live.
+//
+// Row-wise: each row adds m*N with m chosen to clear the lowest position,
+// then drops that position (the window w[] moves down one digit per row, so
+// after L rows it holds the result).  A carry digit may reach 2^28 until the
+// next row normalises it.  The row loop is not unrolled: L^2 multiply-adds
+// once per test (or per checking squaring of a strong PRP) cost nothing
+// measurable, and one copy of a row keeps the unrolled exponentiation's
+// register allocation undisturbed.
+template <int L>
+__device__ __forceinline__ uint32_t redc_class(const uint32_t (&x)[L], const uint32_t (&n)[L], uint32_t np) {
+    uint32_t w[L];
+#pragma unroll
+    for (int j = 0; j < L; ++j) w[j] = x[j];
+#pragma unroll 1
+    for (int i = 0; i < L; ++i) {
+        const uint32_t m = (w[0] * np) & MASK;
+        uint64_t c = mad(m, n[0], w[0]) >> 28;  // the low 28 bits cancel
+#pragma unroll
+        for (int j = 1; j < L; ++j) {
+            const uint64_t v = mad(m, n[j], (uint64_t)w[j] + c);
+            // the digit is cut off before the carry moves on, through an
+            // opaque value: otherwise the cut is deferred to the end of the
+            // row and every 64-bit sum of the row stays live until then
+            uint32_t d = (uint32_t)v & MASK;
+            asm("" : "+v"(d));
+            w[j - 1] = d;
+            c = (v - d) >> 28;
+        }
+        w[L - 1] = (uint32_t)c;
+    }
+    uint32_t d1 = w[0] ^ 1u, dm = w[0] ^ (n[0] - 1u);  // N odd: N - 1 borrows nothing
+#pragma unroll
+    for (int j = 1; j < L; ++j) {
+        d1 |= w[j];
+        dm |= w[j] ^ n[j];
+    }
+    return d1 == 0 ? 1u : (dm == 0 ? 2u : 0u);
+}
+
 // a <- a^2 * R^{-1} * 2^s mod N (s in {0,1}), lazy: a < 4N in, a < 4N out.
 // The 2^s factor is folded into the high (result) columns, so the doubling
 // step of a base-2 exponentiation costs two VALU slots per digit of output.

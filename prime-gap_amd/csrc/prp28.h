@@ -35,14 +35,44 @@ __device__ __forceinline__ void mont_one(const uint32_t (&n)[L], int b, uint32_t
     for (int c = 0; c < cnt; ++c) mod_double<L>(one, n);
 }
 
-// Shared set-up: Mont(1) = R mod N and Mont(2) = 2R mod N.
+// Parking: a lane's column of L words in LDS, stride PARK_STRIDE (one column
+// per thread of a 256-thread block; consecutive lanes hit consecutive banks).
+// In the wide variants (L > 62) the result of an exponentiation loop leaves
+// it through the park and the code after the loop reads x and N again (N
+// from global memory).  With x and N live out of the fully unrolled loop
+// into the check, the register allocator spreads the loop itself over AGPRs
+// (78 digits: 439 registers, 182 ms for 2^18 Fermat tests; parked: 166 ms).
+// The narrow variants keep the check in registers: parked, the 40-digit
+// Fermat kernel measured 1.3% slower.
+constexpr int PARK_STRIDE = 256;
+
+// Whether fermat2<L> uses the park (sprp always does).
+constexpr bool fermat_parks(int L) { return L > 62; }
+
 template <int L>
-__device__ __forceinline__ void mont_one_two(const uint32_t (&n)[L], int b, uint32_t (&one)[L],
-                                             uint32_t (&two)[L]) {
-    mont_one<L>(n, b, one);
+__device__ __forceinline__ void park_store(uint32_t *park, const uint32_t (&x)[L]) {
 #pragma unroll
-    for (int j = 0; j < L; ++j) two[j] = one[j];
-    mod_double<L>(two, n);
+    for (int j = 0; j < L; ++j) park[j * PARK_STRIDE] = x[j];
+}
+
+// x and N from the park and the number's words; the memory clobber keeps the
+// compiler from forwarding the stored values or reusing earlier loads.
+template <int L>
+__device__ __forceinline__ void unpark(const uint32_t *park, const uint32_t *__restrict__ nw, uint32_t limbs,
+                                       uint32_t (&x)[L], uint32_t (&n)[L]) {
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int j = 0; j < L; ++j) x[j] = park[j * PARK_STRIDE];
+    load_n<L>(nw, limbs, n);
+}
+
+// redc_class of the parked value.
+template <int L>
+__device__ __forceinline__ uint32_t parked_class(const uint32_t *park, const uint32_t *__restrict__ nw,
+                                                 uint32_t limbs) {
+    uint32_t x[L], n[L];
+    unpark<L>(park, nw, limbs, x, n);
+    return redc_class<L>(x, n, neg_inv28(n[0]));
 }
 
 // One exponent step x <- x^2 R^{-1} 2^s: the column-folded doubling up to 62
@@ -73,15 +103,14 @@ __device__ __forceinline__ void mul_small(uint32_t (&x)[L], uint32_t b) {
 // Fermat base 2.  The top bit of E = N-1 is consumed by starting at Mont(2);
 // every further bit is one squaring with the doubling folded in.
 template <int L>
-__device__ __forceinline__ uint32_t fermat2(const uint32_t* __restrict__ nw, uint32_t limbs) {
+__device__ __forceinline__ uint32_t fermat2(const uint32_t* __restrict__ nw, uint32_t limbs, uint32_t* park) {
     uint32_t n[L];
     load_n<L>(nw, limbs, n);
     if ((n[0] & 1u) == 0) return 0;
     const int b = bitlen<L>(n);
     if (b < 2) return 0;
     const uint32_t np = neg_inv28(n[0]);
-    // x = Mont(2); Mont(1) is rebuilt after the loop rather than kept, which
-    // frees L registers for the whole exponentiation
+    // x = Mont(2)
     uint32_t x[L];
     mont_one<L>(n, b, x);
     mod_double<L>(x, n);
@@ -96,23 +125,17 @@ __device__ __forceinline__ uint32_t fermat2(const uint32_t* __restrict__ nw, uin
         ew = nxt;
         --i;
     }
-    cond_sub<L>(x, n);
-    uint32_t one[L];
-    mont_one<L>(n, b, one);
-    return equal<L>(x, one) ? 1u : 0u;
-}
-
-// c = (a + b) mod N for a, b < N.
-template <int L>
-__device__ __forceinline__ void mod_add(uint32_t (&a)[L], const uint32_t (&b)[L], const uint32_t (&n)[L]) {
-    uint32_t c = 0;
-#pragma unroll
-    for (int j = 0; j < L; ++j) {
-        uint32_t v = a[j] + b[j] + c;
-        c = v >> 28;
-        a[j] = v & MASK;
+    if constexpr (fermat_parks(L)) {
+        // x = Mont(1) is read from the residue itself (redc_class)
+        park_store<L>(park, x);
+        return parked_class<L>(park, nw, limbs) == 1u ? 1u : 0u;
+    } else {
+        // Mont(1) is rebuilt after the loop rather than held through it
+        cond_sub<L>(x, n);
+        uint32_t one[L];
+        mont_one<L>(n, b, one);
+        return equal<L>(x, one) ? 1u : 0u;
     }
-    cond_sub<L>(a, n);
 }
 
 // Strong probable prime test to one base b < 64 (N - 1 = d * 2^tz):
@@ -122,7 +145,8 @@ __device__ __forceinline__ void mod_add(uint32_t (&a)[L], const uint32_t (&b)[L]
 // Fermat loop), then at most tz - 1 checking squarings, the only place that
 // needs Mont(1) and Mont(-1).
 template <int L>
-__device__ __forceinline__ uint32_t sprp(const uint32_t* __restrict__ nw, uint32_t limbs, uint32_t base) {
+__device__ __forceinline__ uint32_t sprp(const uint32_t* __restrict__ nw, uint32_t limbs, uint32_t base,
+                                         uint32_t* park) {
     uint32_t n[L];
     load_n<L>(nw, limbs, n);
     if ((n[0] & 1u) == 0) return 0;
@@ -136,21 +160,11 @@ __device__ __forceinline__ uint32_t sprp(const uint32_t* __restrict__ nw, uint32
     uint32_t bb = base;
     if (b <= 32) bb = base % nw[0];  // N < 2^32: reduce the base
     if (bb == 0) return 0;           // x = 0 is never +-1
+    // x = bb * Mont(1), congruent to Mont(bb) = bb * R mod N and < bb * N,
+    // which the first squaring absorbs (see mul_small)
     uint32_t x[L];
-    {
-        // x = Mont(bb) = bb * R mod N: Mont(2) directly, else double-and-add
-        // over the bits of bb from Mont(1)
-        uint32_t one[L];
-        mont_one_two<L>(n, b, one, x);
-        if (bb != 2) {
-#pragma unroll
-            for (int j = 0; j < L; ++j) x[j] = 0;
-            for (int t = 5; t >= 0; --t) {
-                mod_double<L>(x, n);
-                if ((bb >> t) & 1u) mod_add<L>(x, one, n);
-            }
-        }
-    }
+    mont_one<L>(n, b, x);
+    mul_small<L>(x, bb);
 
     // Phase 1: exponent bits of d below its top bit, positions b-2 .. tz of
     // N - 1.  Base 2 folds the multiply into the squaring; other bases
@@ -168,37 +182,21 @@ __device__ __forceinline__ uint32_t sprp(const uint32_t* __restrict__ nw, uint32
         --i;
     }
 
-    // Phase 2: x = Mont(b^d) < 2*bb*N.  Bring it below N, then compare with
-    // Mont(+-1) and square at most tz - 1 times looking for Mont(-1).
-    for (int r = 0; r < 128; ++r) {
-        uint32_t br = 0, u[L];
-#pragma unroll
-        for (int j = 0; j < L; ++j) {
-            uint32_t v = x[j] - n[j] - br;
-            br = v >> 31;
-            u[j] = v & MASK;
-        }
-        if (br) break;
-#pragma unroll
-        for (int j = 0; j < L; ++j) x[j] = u[j];
-    }
-    uint32_t one[L], mone[L];  // Mont(1), Mont(-1) = N - Mont(1)
-    mont_one<L>(n, b, one);
-    {
-        uint32_t br = 0;
-#pragma unroll
-        for (int j = 0; j < L; ++j) {
-            uint32_t v = n[j] - one[j] - br;
-            br = v >> 31;
-            mone[j] = v & MASK;
-        }
-    }
-    if (equal<L>(x, one) || equal<L>(x, mone)) return 1;
+    // Phase 2: x = Mont(b^d) < 2*bb*N.  Pass if b^d = +-1, else square at
+    // most tz - 1 times looking for -1 (redc_class reads the residue of the
+    // lazy Montgomery form directly).  x lives in the park between steps.
+    park_store<L>(park, x);
+    if (parked_class<L>(park, nw, limbs) != 0) return 1;
     for (int c = 1; c < tz; ++c) {
-        sqr_step<L>(x, n, np, 0u);
-        cond_sub<L>(x, n);  // x < 2N -> x < N
-        if (equal<L>(x, mone)) return 1;
-        if (equal<L>(x, one)) return 0;  // 1 without passing -1: composite
+        {
+            uint32_t y[L], m[L];
+            unpark<L>(park, nw, limbs, y, m);
+            sqr_step<L>(y, m, neg_inv28(m[0]), 0u);
+            park_store<L>(park, y);
+        }
+        const uint32_t k = parked_class<L>(park, nw, limbs);
+        if (k == 2) return 1;
+        if (k == 1) return 0;  // 1 without passing -1: composite
     }
     return 0;
 }

@@ -738,10 +738,17 @@ class Scanner {
         // bound per lane), so every round is sized to the numbers the chosen
         // variant keeps resident at once: few active sides get a deep
         // speculative window, many get one or two tests each.
+        //
+        // When the variant keeps more than one wavefront per SIMD resident,
+        // a full round takes that many tests' time; while the scanning sides
+        // are few (at most a quarter of one wave per SIMD's lanes) the round
+        // is held to one wave per SIMD, so it costs one test's latency and
+        // each side still gets a deep window.
         const Variant *vr = vc ? vc : vf;
         int (*occ)() = vc ? vc->scan_blocks_per_cu : vf->blocks_per_cu;
-        const uint32_t resident =
-            (uint32_t)std::max(1, occ ? occ() : 1) * e->num_cus * 256u / (uint32_t)std::max(1, vr->lanes);
+        const uint32_t per_wave = e->num_cus * 256u / (uint32_t)std::max(1, vr->lanes);
+        const uint32_t resident_full = (uint32_t)std::max(1, occ ? occ() : 1) * per_wave;
+        const uint32_t resident = 2ull * na >= per_wave / 4 ? resident_full : std::min(resident_full, per_wave);
         const uint32_t cpad = 64u / (uint32_t)std::max(1, vr->lanes);  // items per wavefront
         Bases none{}, conf{};
         static const uint32_t mr_bases[kMaxBases] = {3, 5, 7, 11, 13, 17, 19, 23};
