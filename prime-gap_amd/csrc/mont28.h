@@ -208,11 +208,11 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&a)[L], const uint32_t (&n)[L
 // digits (a one-bit shift with carry) instead of to the column sums, whose
 // extra bit would break the 64-bit column bound above 63 digits.
 //
-// The wide variants run one wavefront per SIMD (their registers leave room for
-// no second one), so nothing hides the latency of a dependent multiply-add.
-// Each column is therefore summed in three independent chains -- the cross
-// products, and the m*N products in two halves -- issued round-robin, and
-// merged once per column (exact integer sums: the result is unchanged).
+// One dependent chain per column, as in the narrow form.  (While the wide
+// kernels kept the check's registers live out of the loop and ran one
+// wavefront per SIMD, three independent chains per column were faster; with
+// the parked check, 78 digits, one chain: 161.7 ms per 2^18 Fermat tests at
+// two waves per SIMD against 166.7 ms with three.)
 template <int L>
 __device__ __forceinline__ void sqr_redc_wide(uint32_t (&a)[L], const uint32_t (&n)[L], uint32_t np,
                                               uint32_t s) {
@@ -222,29 +222,21 @@ __device__ __forceinline__ void sqr_redc_wide(uint32_t (&a)[L], const uint32_t (
 #pragma unroll
     for (int k = 0; k < 2 * L - 1; ++k) {
         const int ilo = k - (L - 1) > 0 ? k - (L - 1) : 0;
-        const int xe = (k + 1) / 2;        // cross products a_i a_{k-i}, i in [ilo, xe)
-        const int ye = k < L ? k : L;      // m_i n_{k-i}, i in [ilo, ye) (m_k joins below)
-        const int ym = (ilo + ye + 1) / 2; // split point of the m*N chain
-        const int nx = xe - ilo, n0 = ym - ilo, n1 = ye - ym;
-        const int rounds = nx > n0 ? (nx > n1 ? nx : n1) : (n0 > n1 ? n0 : n1);
-        uint64_t X = 0, Y0 = T, Y1 = 0;
+        // cross products a_i * a_{k-i}, i < k-i
+        uint64_t X = 0;
 #pragma unroll
-        for (int r = 0; r < rounds; ++r) {
-            if (r < nx) X = mad(a[ilo + r], a[k - ilo - r], X);
-            if (r < n0) Y0 = mad(m[ilo + r], n[k - ilo - r], Y0);
-            if (r < n1) Y1 = mad(m[ym + r], n[k - ym - r], Y1);
-        }
-        // opaque to the optimiser: otherwise reassociation folds the three
-        // sums back into one dependent chain
-        asm("" : "+v"(X), "+v"(Y0), "+v"(Y1));
-        T = (X << 1) + Y0;
-        T += Y1;
+        for (int i = ilo; 2 * i < k; ++i) X = mad(a[i], a[k - i], X);
+        T = (X << 1) + T;
         if ((k & 1) == 0) T = mad(a[k >> 1], a[k >> 1], T);
+        const int ye = k < L ? k : L;  // m_i n_{k-i}, i < ye (m_k joins below)
+#pragma unroll
+        for (int i = ilo; i < ye; ++i) T = mad(m[i], n[k - i], T);
         if (k < L) {
             m[k] = ((uint32_t)T * np) & MASK;
             T = mad(m[k], n[0], T);
             T >>= 28;
         } else {
+            // digit k-L of the result; a[k-L] and m[k-L] are dead here
             a[k - L] = (uint32_t)T & MASK;
             T >>= 28;
         }
