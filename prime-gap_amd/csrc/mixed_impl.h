@@ -31,9 +31,11 @@ constexpr size_t kMixedParkBytes =
 template <int KIND, int L>
 __device__ __forceinline__ uint32_t test_one(const uint32_t *nw, uint32_t limbs, const Bases &bases,
                                              uint32_t *park) {
-    if (KIND == KIND_FERMAT2) return r28::fermat2<L>(nw, limbs, park);
+    // the persistent kernel runs one wavefront per SIMD (its widest bucket
+    // sets the registers): three chains per column in the wide squaring
+    if (KIND == KIND_FERMAT2) return r28::fermat2<L, 3>(nw, limbs, park);
     uint32_t v = 1;
-    for (uint32_t k = 0; k < bases.n && v; ++k) v = r28::sprp<L>(nw, limbs, bases.b[k], park);
+    for (uint32_t k = 0; k < bases.n && v; ++k) v = r28::sprp<L, 3>(nw, limbs, bases.b[k], park);
     return v;
 }
 

@@ -208,29 +208,54 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&a)[L], const uint32_t (&n)[L
 // digits (a one-bit shift with carry) instead of to the column sums, whose
 // extra bit would break the 64-bit column bound above 63 digits.
 //
-// One dependent chain per column, as in the narrow form.  (While the wide
-// kernels kept the check's registers live out of the loop and ran one
-// wavefront per SIMD, three independent chains per column were faster; with
-// the parked check, 78 digits, one chain: 161.7 ms per 2^18 Fermat tests at
-// two waves per SIMD against 166.7 ms with three.)
-template <int L>
+// C = chains per column.  C = 1: one dependent chain, as in the narrow form,
+// for kernels that keep two wavefronts per SIMD (78 digits, parked check,
+// 2^18 Fermat tests: 161.7 ms against 166.7 ms with three chains).  C = 3:
+// the cross products and the m*N products in two halves summed in
+// independent chains, issued round-robin and merged once per column, for
+// kernels that run one wavefront per SIMD, where nothing else hides the
+// latency of a dependent multiply-add (1511# streaming scan: 11.3 k m/s
+// against 10.0 k with one chain; persistent mixed-width batch 2.88 M tests/s
+// against 2.80 M).  Exact integer sums: the result does not depend on C.
+template <int L, int C = 1>
 __device__ __forceinline__ void sqr_redc_wide(uint32_t (&a)[L], const uint32_t (&n)[L], uint32_t np,
                                               uint32_t s) {
     static_assert(L >= 2 && L <= 127, "radix-2^28 column bound");
+    static_assert(C == 1 || C == 3, "one or three chains per column");
     uint32_t m[L];
     uint64_t T = 0;
 #pragma unroll
     for (int k = 0; k < 2 * L - 1; ++k) {
         const int ilo = k - (L - 1) > 0 ? k - (L - 1) : 0;
-        // cross products a_i * a_{k-i}, i < k-i
-        uint64_t X = 0;
+        const int ye = k < L ? k : L;  // m_i n_{k-i}, i in [ilo, ye) (m_k joins below)
+        if constexpr (C == 1) {
+            // cross products a_i * a_{k-i}, i < k-i
+            uint64_t X = 0;
 #pragma unroll
-        for (int i = ilo; 2 * i < k; ++i) X = mad(a[i], a[k - i], X);
-        T = (X << 1) + T;
-        if ((k & 1) == 0) T = mad(a[k >> 1], a[k >> 1], T);
-        const int ye = k < L ? k : L;  // m_i n_{k-i}, i < ye (m_k joins below)
+            for (int i = ilo; 2 * i < k; ++i) X = mad(a[i], a[k - i], X);
+            T = (X << 1) + T;
+            if ((k & 1) == 0) T = mad(a[k >> 1], a[k >> 1], T);
 #pragma unroll
-        for (int i = ilo; i < ye; ++i) T = mad(m[i], n[k - i], T);
+            for (int i = ilo; i < ye; ++i) T = mad(m[i], n[k - i], T);
+        } else {
+            const int xe = (k + 1) / 2;         // cross products a_i a_{k-i}, i in [ilo, xe)
+            const int ym = (ilo + ye + 1) / 2;  // split point of the m*N chain
+            const int nx = xe - ilo, n0 = ym - ilo, n1 = ye - ym;
+            const int rounds = nx > n0 ? (nx > n1 ? nx : n1) : (n0 > n1 ? n0 : n1);
+            uint64_t X = 0, Y0 = T, Y1 = 0;
+#pragma unroll
+            for (int r = 0; r < rounds; ++r) {
+                if (r < nx) X = mad(a[ilo + r], a[k - ilo - r], X);
+                if (r < n0) Y0 = mad(m[ilo + r], n[k - ilo - r], Y0);
+                if (r < n1) Y1 = mad(m[ym + r], n[k - ym - r], Y1);
+            }
+            // opaque to the optimiser: otherwise reassociation folds the three
+            // sums back into one dependent chain
+            asm("" : "+v"(X), "+v"(Y0), "+v"(Y1));
+            T = (X << 1) + Y0;
+            T += Y1;
+            if ((k & 1) == 0) T = mad(a[k >> 1], a[k >> 1], T);
+        }
         if (k < L) {
             m[k] = ((uint32_t)T * np) & MASK;
             T = mad(m[k], n[0], T);

@@ -7,17 +7,16 @@
 //
 // Squaring is row-wise Montgomery (CIOS): for i = 0..L-1
 //     T += x_i * x;   m = T_0 * n' mod 2^26;   T += m * N;   T /= 2^26
-// x_i and m are broadcast inside the group with DPP (a modifier on a v_mov,
-// no LDS): quad_perm within a quad, plus a row shift under a bank mask for
-// groups of 8.  T is kept as 64-bit unnormalised digit accumulators; a
+// m is broadcast inside the group with DPP (a modifier on a v_mov): quad_perm
+// within a quad, plus a row shift under a bank mask for groups of 8; the row
+// multipliers x_i are read from an LDS stage (see sqr_redc).  T is kept as 64-bit unnormalised digit accumulators; a
 // 26x26-bit product is < 2^52 and each accumulator takes two per row for at
 // most L = 168 rows, < 2^61, so no carry moves until the row loop ends.  The
 // division by 2^26 moves every accumulator down one position; positions are
 // mapped to registers with a rotating offset (position p of row i lives in
 // register (i + p) mod S), so the shift costs one 64-bit DPP move from the
-// next lane, not S register moves.  The multiplier digits x_i come from a
-// copy of x that rotates one lane down after every S rows, so the broadcast
-// source is always lane 0 of the group and the S-row body is unrolled once.
+// next lane, not S register moves.  The S rows that multiply by one lane's
+// digits share one unrolled body.
 //
 // G = 8 halves the registers per lane (5S + temporaries), which lets a
 // 4247-bit number run at more than one wavefront per SIMD without the
@@ -208,21 +207,37 @@ __device__ __forceinline__ bool equal(const uint32_t (&a)[S], const uint32_t (&b
     return gor<G>(d) == 0;
 }
 
+// Row multipliers through LDS.  Each squaring, lane g of a group writes its
+// S digits to the group's stage (row g, stride stage_stride(S) words); the S
+// rows that use lane o's digits then read them with one broadcast load each
+// (every lane of the group reads the same word), in place of a two-step DPP
+// broadcast plus its operand copy per row on the VALU.  The stage is private
+// to one group of one wavefront, whose LDS operations complete in order, so
+// no barrier is needed.  The odd stride puts the groups of a wavefront in
+// different banks.
+constexpr int stage_stride(int S) { return S | 1; }
+constexpr size_t stage_bytes(int S) { return (size_t)256 * stage_stride(S) * sizeof(uint32_t); }
+
 // x <- x^2 R^{-1} mod N (lazy: x < 4N in, < 2N out), R = 2^(26*G*S).
+// `stage` is the group's LDS stage (G rows of stage_stride(S) words).
 template <int S, int G>
-__device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t np, uint32_t g) {
+__device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t np, uint32_t g,
+                                         uint32_t *stage) {
+    constexpr int ST = stage_stride(S);
     uint64_t t[S];
-    uint32_t xs[S];
 #pragma unroll
     for (int p = 0; p < S; ++p) {
         t[p] = 0;
-        xs[p] = x[p];
+        stage[g * ST + p] = x[p];
     }
     const bool lane0 = (g == 0), top = (g == G - 1);
     for (int o = 0; o < G; ++o) {
+        uint32_t xb[S];
+#pragma unroll
+        for (int u = 0; u < S; ++u) xb[u] = stage[o * ST + u];
 #pragma unroll
         for (int u = 0; u < S; ++u) {
-            const uint32_t xi = gbcast0<G>(xs[u]);
+            const uint32_t xi = xb[u];
 #pragma unroll
             for (int j = 0; j < S; ++j) t[(u + j) % S] = mad(xi, x[j], t[(u + j) % S]);
             const uint32_t m = gbcast0<G>(((uint32_t)t[u] * np) & MASK);
@@ -233,9 +248,6 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S
             t[(u + 1) % S] += c;
             t[u] = top ? 0ull : in;
         }
-        // the next S rows broadcast the next lane's digits
-#pragma unroll
-        for (int u = 0; u < S; ++u) xs[u] = gdown<G>(xs[u]);
     }
     normalize<S, G>(t, x, g);
 }

@@ -77,12 +77,14 @@ __device__ __forceinline__ uint32_t parked_class(const uint32_t *park, const uin
 
 // One exponent step x <- x^2 R^{-1} 2^s: the column-folded doubling up to 62
 // digits, the digit-pass doubling beyond (the column bound of the folded form).
-template <int L>
+// C: chains per column of the wide form (1, or 3 for kernels that run one
+// wavefront per SIMD).
+template <int L, int C = 1>
 __device__ __forceinline__ void sqr_step(uint32_t (&x)[L], const uint32_t (&n)[L], uint32_t np, uint32_t s) {
     if constexpr (L <= 62)
         sqr_redc<L>(x, n, np, s);
     else
-        sqr_redc_wide<L>(x, n, np, s);
+        sqr_redc_wide<L, C>(x, n, np, s);
 }
 
 // x <- x * b for a small base b (b < 64): digits times b with the carry
@@ -102,7 +104,7 @@ __device__ __forceinline__ void mul_small(uint32_t (&x)[L], uint32_t b) {
 
 // Fermat base 2.  The top bit of E = N-1 is consumed by starting at Mont(2);
 // every further bit is one squaring with the doubling folded in.
-template <int L>
+template <int L, int C = 1>
 __device__ __forceinline__ uint32_t fermat2(const uint32_t* __restrict__ nw, uint32_t limbs, uint32_t* park) {
     uint32_t n[L];
     load_n<L>(nw, limbs, n);
@@ -121,7 +123,7 @@ __device__ __forceinline__ uint32_t fermat2(const uint32_t* __restrict__ nw, uin
         const uint32_t s = (i == 0) ? 0u : ((ew >> (i & 31)) & 1u);
         uint32_t nxt = ew;
         if ((i & 31) == 0 && i > 0) nxt = nw[(i >> 5) - 1];
-        sqr_step<L>(x, n, np, s);
+        sqr_step<L, C>(x, n, np, s);
         ew = nxt;
         --i;
     }
@@ -144,7 +146,7 @@ __device__ __forceinline__ uint32_t fermat2(const uint32_t* __restrict__ nw, uin
 // holds only x, N and the squaring's temporaries (so it runs as fast as the
 // Fermat loop), then at most tz - 1 checking squarings, the only place that
 // needs Mont(1) and Mont(-1).
-template <int L>
+template <int L, int C = 1>
 __device__ __forceinline__ uint32_t sprp(const uint32_t* __restrict__ nw, uint32_t limbs, uint32_t base,
                                          uint32_t* park) {
     uint32_t n[L];
@@ -176,7 +178,7 @@ __device__ __forceinline__ uint32_t sprp(const uint32_t* __restrict__ nw, uint32
         const uint32_t bit = (ew >> (i & 31)) & 1u;
         uint32_t nxt = ew;
         if ((i & 31) == 0 && i > tz) nxt = nw[(i >> 5) - 1];
-        sqr_step<L>(x, n, np, bb == 2 ? bit : 0u);
+        sqr_step<L, C>(x, n, np, bb == 2 ? bit : 0u);
         if (bb != 2 && bit) mul_small<L>(x, bb);
         ew = nxt;
         --i;
@@ -191,7 +193,7 @@ __device__ __forceinline__ uint32_t sprp(const uint32_t* __restrict__ nw, uint32
         {
             uint32_t y[L], m[L];
             unpark<L>(park, nw, limbs, y, m);
-            sqr_step<L>(y, m, neg_inv28(m[0]), 0u);
+            sqr_step<L, C>(y, m, neg_inv28(m[0]), 0u);
             park_store<L>(park, y);
         }
         const uint32_t k = parked_class<L>(park, nw, limbs);

@@ -53,7 +53,8 @@ __device__ __forceinline__ void mont_one_two(const uint32_t (&n)[S], uint32_t b,
 }
 
 template <int S, int G>
-__device__ __forceinline__ uint32_t fermat2(const uint32_t *__restrict__ nw, uint32_t limbs, uint32_t g) {
+__device__ __forceinline__ uint32_t fermat2(const uint32_t *__restrict__ nw, uint32_t limbs, uint32_t g,
+                                            uint32_t *stage) {
     uint32_t n[S];
     load_digits<S, G>(nw, limbs, g, n);
     const uint32_t n0 = gbcast0<G>(n[0]);
@@ -71,7 +72,7 @@ __device__ __forceinline__ uint32_t fermat2(const uint32_t *__restrict__ nw, uin
         const uint32_t s = (i == 0) ? 0u : ((ew >> (i & 31)) & 1u);
         uint32_t nxt = ew;
         if ((i & 31) == 0 && i > 0) nxt = nw[(i >> 5) - 1];
-        sqr_redc<S, G>(x, n, np, g);
+        sqr_redc<S, G>(x, n, np, g, stage);
         shl<S, G>(x, s, g);
         ew = nxt;
         --i;
@@ -104,7 +105,7 @@ __device__ __forceinline__ void mod_add(uint32_t (&a)[S], const uint32_t (&c)[S]
 
 template <int S, int G>
 __device__ __forceinline__ uint32_t sprp(const uint32_t *__restrict__ nw, uint32_t limbs, uint32_t base,
-                                         uint32_t g) {
+                                         uint32_t g, uint32_t *stage) {
     uint32_t n[S];
     load_digits<S, G>(nw, limbs, g, n);
     const uint32_t n0 = gbcast0<G>(n[0]);
@@ -138,7 +139,7 @@ __device__ __forceinline__ uint32_t sprp(const uint32_t *__restrict__ nw, uint32
         const uint32_t bit = (ew >> (i & 31)) & 1u;
         uint32_t nxt = ew;
         if ((i & 31) == 0 && i > tz) nxt = nw[(i >> 5) - 1];
-        sqr_redc<S, G>(x, n, np, g);
+        sqr_redc<S, G>(x, n, np, g, stage);
         shl<S, G>(x, bb == 2 ? bit : 0u, g);
         if (bb != 2 && bit) mul_small<S, G>(x, bb, g);
         ew = nxt;
@@ -157,7 +158,7 @@ __device__ __forceinline__ uint32_t sprp(const uint32_t *__restrict__ nw, uint32
     (void)sub_n<S, G>(n, one, mone, g);  // N - Mont(1): no borrow since Mont(1) < N
     if (equal<S, G>(x, one) || equal<S, G>(x, mone)) return 1;
     for (int c = 1; c < tz; ++c) {
-        sqr_redc<S, G>(x, n, np, g);
+        sqr_redc<S, G>(x, n, np, g, stage);
         cond_sub<S, G>(x, n, g);  // x < 2N -> x < N
         if (equal<S, G>(x, mone)) return 1;
         if (equal<S, G>(x, one)) return 0;

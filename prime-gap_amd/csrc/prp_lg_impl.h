@@ -1,10 +1,27 @@
 // prp_lg_impl.h -- batch kernels for the lane-group radix-2^26 variants (G = 4
 // or 8 lanes per number); included by the prp_lg_*.hip instantiation units.
 #pragma once
+#include <algorithm>
 #include "kernels.h"
 #include "prp_lg.h"
 
 namespace pgap {
+
+// Dynamic LDS of a lane-group launch: the squaring's stage, or the occupancy
+// cap's allocation when that is larger.
+template <typename K>
+size_t lg_lds_bytes(K kernel, int S, int blocks_cap) {
+    const size_t bytes = std::max(lg::stage_bytes(S), occupancy_cap_bytes(kernel, blocks_cap));
+    if (bytes > 64 * 1024)
+        (void)hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    return bytes;
+}
+
+// This thread's group stage in the block's dynamic LDS.
+template <int S, int G>
+__device__ __forceinline__ uint32_t *group_stage(uint32_t *lds) {
+    return lds + (threadIdx.x / G) * G * lg::stage_stride(S);
+}
 
 // G lanes per number: item = thread / G.  A group returns together (its
 // lanes share the item), so DPP exchanges never read a finished lane of the
@@ -14,6 +31,8 @@ __global__ void __launch_bounds__(256) prp_lg_kernel(const uint32_t *__restrict_
                                                      const uint32_t *__restrict__ d_count,
                                                      const uint32_t *__restrict__ d_idx, Bases bases,
                                                      uint8_t *__restrict__ out) {
+    extern __shared__ uint32_t lg_lds[];
+    uint32_t *stage = group_stage<S, G>(lg_lds);
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     const uint32_t item = t / G, g = t % G;
     const uint32_t cnt = d_count ? min(*d_count, count) : count;
@@ -22,10 +41,10 @@ __global__ void __launch_bounds__(256) prp_lg_kernel(const uint32_t *__restrict_
     const uint32_t *nw = n + (size_t)row * limbs;
     uint32_t v;
     if (KIND == KIND_FERMAT2) {
-        v = lg::fermat2<S, G>(nw, limbs, g);
+        v = lg::fermat2<S, G>(nw, limbs, g, stage);
     } else {
         v = 1;
-        for (uint32_t k = 0; k < bases.n && v; ++k) v = lg::sprp<S, G>(nw, limbs, bases.b[k], g);
+        for (uint32_t k = 0; k < bases.n && v; ++k) v = lg::sprp<S, G>(nw, limbs, bases.b[k], g, stage);
     }
     if (g == 0) out[row] = (uint8_t)v;
 }
@@ -35,7 +54,7 @@ hipError_t launch_lg(const uint32_t *d_n, uint32_t limbs, uint32_t count, const 
                      const uint32_t *d_idx, Bases bases, uint8_t *d_out, int blocks_cap, hipStream_t stream) {
     if (count == 0) return hipSuccess;
     const dim3 grid((uint32_t)(((size_t)count * G + 255) / 256));
-    const size_t lds = occupancy_cap_bytes(prp_lg_kernel<S, G, KIND>, blocks_cap);
+    const size_t lds = lg_lds_bytes(prp_lg_kernel<S, G, KIND>, S, blocks_cap);
     hipLaunchKernelGGL((prp_lg_kernel<S, G, KIND>), grid, dim3(256), lds, stream, d_n, limbs, count, d_count, d_idx,
                        bases, d_out);
     return hipGetLastError();
@@ -48,6 +67,8 @@ __global__ void __launch_bounds__(256) scan_lg_kernel(const uint32_t *__restrict
                                                       uint32_t conf_rows, uint32_t conf_max, uint32_t max_items,
                                                       const uint32_t *__restrict__ ctr,
                                                       const uint8_t *__restrict__ base, uint8_t *__restrict__ out) {
+    extern __shared__ uint32_t lg_lds[];
+    uint32_t *stage = group_stage<S, G>(lg_lds);
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     const uint32_t item = t / G, g = t % G;
     const uint32_t nc = min(ctr[3], conf_max), nf = min(ctr[0], max_items);
@@ -57,11 +78,11 @@ __global__ void __launch_bounds__(256) scan_lg_kernel(const uint32_t *__restrict
     if (item < ncw) {
         if (item >= nc) return;
         row = item;
-        v = lg::sprp<S, G>(n + (size_t)row * limbs, limbs, base[item], g);
+        v = lg::sprp<S, G>(n + (size_t)row * limbs, limbs, base[item], g, stage);
     } else {
         if (item - ncw >= nf) return;
         row = conf_rows + (item - ncw);
-        v = lg::fermat2<S, G>(n + (size_t)row * limbs, limbs, g);
+        v = lg::fermat2<S, G>(n + (size_t)row * limbs, limbs, g, stage);
     }
     if (g == 0) out[row] = (uint8_t)v;
 }
@@ -74,7 +95,7 @@ hipError_t launch_scan_lg(const uint32_t *d_n, uint32_t limbs, uint32_t conf_row
     const size_t items = ((size_t)conf_max + per_wave - 1) / per_wave * per_wave + max_items;
     if (items == 0) return hipSuccess;
     const dim3 grid((uint32_t)((items * G + 255) / 256));
-    const size_t lds = occupancy_cap_bytes(scan_lg_kernel<S, G>, blocks_cap);
+    const size_t lds = lg_lds_bytes(scan_lg_kernel<S, G>, S, blocks_cap);
     hipLaunchKernelGGL((scan_lg_kernel<S, G>), grid, dim3(256), lds, stream, d_n, limbs, conf_rows, conf_max,
                        max_items, d_ctr, d_base, d_out);
     return hipGetLastError();
@@ -83,7 +104,8 @@ hipError_t launch_scan_lg(const uint32_t *d_n, uint32_t limbs, uint32_t conf_row
 template <int S, int G>
 int occupancy_scan_lg() {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_lg_kernel<S, G>, 256, 0) != hipSuccess || n <= 0)
+    const size_t lds = lg_lds_bytes(scan_lg_kernel<S, G>, S, 0);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_lg_kernel<S, G>, 256, lds) != hipSuccess || n <= 0)
         n = 1;
     return n;
 }
@@ -91,7 +113,9 @@ int occupancy_scan_lg() {
 template <int S, int G, int KIND>
 int occupancy_lg() {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, prp_lg_kernel<S, G, KIND>, 256, 0) != hipSuccess || n <= 0)
+    const size_t lds = lg_lds_bytes(prp_lg_kernel<S, G, KIND>, S, 0);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, prp_lg_kernel<S, G, KIND>, 256, lds) != hipSuccess ||
+        n <= 0)
         n = 1;
     return n;
 }

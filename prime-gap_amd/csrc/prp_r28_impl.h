@@ -17,6 +17,9 @@ namespace pgap {
 constexpr int r28_wmin(int L, bool scan) { return L > 62 ? (scan ? 1 : 2) : 1; }
 constexpr int r28_wmax(int L, bool scan) { return L > 62 ? (scan ? 1 : 2) : 8; }
 
+// Chains per column of the wide squaring in the one-wave scan kernels.
+constexpr int kScanChains = 3;
+
 // LDS park bytes of an r28 kernel (one column of L words per thread).
 constexpr size_t r28_park_bytes(int L, bool parks) {
     return parks ? (size_t)L * r28::PARK_STRIDE * sizeof(uint32_t) : 0;
@@ -84,11 +87,11 @@ scan_r28_kernel(const uint32_t *__restrict__ n, uint32_t limbs, uint32_t conf_ro
     if (t < ncw) {
         if (t >= nc) return;
         row = t;
-        v = r28::sprp<L>(n + (size_t)row * limbs, limbs, base[t], park);
+        v = r28::sprp<L, kScanChains>(n + (size_t)row * limbs, limbs, base[t], park);
     } else {
         if (t - ncw >= nf) return;
         row = conf_rows + (t - ncw);
-        v = r28::fermat2<L>(n + (size_t)row * limbs, limbs, park);
+        v = r28::fermat2<L, kScanChains>(n + (size_t)row * limbs, limbs, park);
     }
     out[row] = (uint8_t)v;
 }
