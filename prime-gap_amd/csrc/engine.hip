@@ -144,35 +144,33 @@ __global__ void zero_kernel(uint32_t *__restrict__ p) {
 
 // Several strong-PRP bases: the first on every row, each later one only on
 // the rows that passed so far (selected on the device), so lanes that
-// already failed do not ride along through the later bases.
+// already failed do not ride along through the later bases.  Every launch
+// tests one base (the kernels' contract).
 hipError_t pgap_engine::enqueue_batch(const Choice &c, const uint32_t *d_n, uint32_t limbs, size_t count,
                                       const Bases &bases, uint8_t *d_out, uint32_t *launches) {
     const Variant *v = c.v;
-    const size_t chunk = (size_t)1 << 30;  // grid dimensions are 32-bit: split very large batches
-    const bool staged = v->kind == KIND_SPRP && bases.n > 1 && count <= chunk;
+    const size_t chunk = kBatchChunk;  // grid dimensions are 32-bit: split very large batches
+    const uint32_t nb = v->kind == KIND_SPRP ? bases.n : 1;
     Bases first = bases;
-    if (staged) first.n = 1;
+    if (v->kind == KIND_SPRP) first.n = 1;
+    uint32_t *sel = (uint32_t *)d_sel.p, *nsel = (uint32_t *)d_nsel.p;
     hipError_t e;
     for (size_t off = 0; off < count; off += chunk) {
         const uint32_t n = (uint32_t)std::min(chunk, count - off);
-        if ((e = v->launch(d_n + off * limbs, limbs, n, nullptr, nullptr, first, d_out + off, c.cap, stream)) !=
-            hipSuccess)
-            return e;
+        const uint32_t *dn = d_n + off * limbs;
+        uint8_t *dout = d_out + off;
+        if ((e = v->launch(dn, limbs, n, nullptr, nullptr, first, dout, c.cap, stream)) != hipSuccess) return e;
         ++*launches;
-    }
-    if (!staged) return hipSuccess;
-    uint32_t *sel = (uint32_t *)d_sel.p, *nsel = (uint32_t *)d_nsel.p;
-    for (uint32_t k = 1; k < bases.n; ++k) {
-        Bases one{};
-        one.n = 1;
-        one.b[0] = bases.b[k];
-        hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, nsel);
-        hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0, stream, d_out,
-                           (uint32_t)count, sel, nsel);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = v->launch(d_n, limbs, (uint32_t)count, nsel, sel, one, d_out, c.cap, stream)) != hipSuccess)
-            return e;
-        ++*launches;
+        for (uint32_t k = 1; k < nb; ++k) {
+            Bases one{};
+            one.n = 1;
+            one.b[0] = bases.b[k];
+            hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, nsel);
+            hipLaunchKernelGGL(select_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, dout, n, sel, nsel);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            if ((e = v->launch(dn, limbs, n, nsel, sel, one, dout, c.cap, stream)) != hipSuccess) return e;
+            ++*launches;
+        }
     }
     return hipSuccess;
 }
@@ -182,7 +180,8 @@ pgap_status pgap_engine::launch_batch(const Choice &c, const uint32_t *d_n, uint
     const Variant *v = c.v;
     if (v->kind == KIND_SPRP && bases.n > 1) {
         pgap_status st;
-        if ((st = ensure(d_sel, count * 4, "survivor index")) || (st = ensure(d_nsel, 64, "survivor count")))
+        if ((st = ensure(d_sel, std::min(count, kBatchChunk) * 4, "survivor index")) ||
+            (st = ensure(d_nsel, 64, "survivor count")))
             return st;
     }
     BatchGraph key;
@@ -271,7 +270,7 @@ static pgap_status check_bases(pgap_engine *e, pgap_test kind, const uint32_t *b
     for (uint32_t i = 0; i < nbases; ++i) {
         // bases are small primes in practice (generate_primes, miller_rabin.h:
         // 269-286); the kernels multiply by the base as a scalar, which the
-        // lazy-reduction bound allows for bases below 64.
+        // lazy-reduction bound allows for bases below 64 (select_bits).
         if (bases[i] < 2 || bases[i] > 63) {
             e->set_error("SPRP base %u outside [2, 63]", bases[i]);
             return PGAP_E_ARG;
@@ -386,7 +385,7 @@ pgap_status pgap_prp_batch(pgap_engine *e, uint32_t limbs, const uint32_t *n, si
     (void)hipSetDevice(e->opt.device);
     const uint32_t bits = host_max_bits(n, limbs, count);
     Choice v;
-    if ((st = e->pick((int)kind, std::max(bits, 2u), &v))) return st;
+    if ((st = e->pick((int)kind, select_bits((int)kind, std::max(bits, 2u), bs), &v))) return st;
     const size_t in_bytes = count * limbs * sizeof(uint32_t);
     if ((st = e->ensure(e->d_n, in_bytes, "numbers")) || (st = e->ensure(e->d_out, count, "verdicts")))
         return st;
@@ -430,7 +429,7 @@ pgap_status pgap_prp_batch_device(pgap_engine *e, uint32_t limbs, const uint32_t
         if (he != hipSuccess) return e->hip_fail(he, "max bits");
     }
     Choice v;
-    if ((st = e->pick((int)kind, std::max(max_bits, 2u), &v))) return st;
+    if ((st = e->pick((int)kind, select_bits((int)kind, std::max(max_bits, 2u), bs), &v))) return st;
     return e->launch_batch(v, d_n, limbs, count, bs, d_verdict);
 }
 

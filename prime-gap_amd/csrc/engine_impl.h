@@ -29,6 +29,9 @@ struct TuneEntry {
     double rate = 0;
 };
 
+// Rows per batch launch: grid dimensions are 32-bit.
+constexpr size_t kBatchChunk = (size_t)1 << 30;
+
 // A variant and the occupancy cap to launch it with.
 struct Choice {
     const Variant *v = nullptr;

@@ -19,7 +19,11 @@ struct Bases {
 
 enum Kind { KIND_FERMAT2 = 0, KIND_SPRP = 1 };
 
-// Batch launch: numbers are rows of `limbs` words; verdicts out.
+// Batch launch: numbers are rows of `limbs` words; verdicts out.  A strong-PRP
+// launch tests one base, bases.b[0] (bases.n must be 1; callers run further
+// bases on the survivors through d_idx): a loop over bases inside the kernel
+// keeps the whole test live across iterations and costs the lane-group
+// kernels a wavefront per SIMD.
 //   count    grid size (upper bound of the work)
 //   d_count  if non-null, the actual count is read from device memory (so a
 //            round can size its work on the device without a host round trip)
@@ -59,12 +63,24 @@ struct Variant {
     int digits;      // digits (radix-sized) per number
     int lanes;       // lanes cooperating on one number
     int kind;        // Kind
-    int max_bits;    // largest N the variant accepts (lazy reduction needs N < R/16)
+    int max_bits;    // largest N the variant accepts: Fermat N < R/16; strong PRP N < R/2^8,
+                     // which covers bases up to 8 (see select_bits)
     batch_launch_fn launch;
     int (*blocks_per_cu)();  // resident 256-thread workgroups per CU (occupancy query)
     scan_launch_fn scan = nullptr;  // set on strong-PRP variants (max_bits is the SPRP limit)
     int (*scan_blocks_per_cu)() = nullptr;  // occupancy of the scan kernel
 };
+
+// Width to select a variant with.  Strong PRP multiplies by the base as a
+// scalar: x < 2N becomes x < 2bN, and the next lazy squaring returns < 2N
+// while 4b^2 N < R.  Variants register max_bits for bases up to 8 (N < R/2^8);
+// a base in [9, 63] needs N < R/2^14, six bits more headroom.
+inline uint32_t select_bits(int kind, uint32_t bits, const Bases &b) {
+    if (kind != KIND_SPRP) return bits;
+    for (uint32_t i = 0; i < b.n; ++i)
+        if (b.b[i] > 8) return bits + 6;
+    return bits;
+}
 
 // Registry: filled by every instantiation unit at library load.
 void register_variant(const Variant &v);

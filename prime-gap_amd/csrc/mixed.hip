@@ -71,7 +71,8 @@ static pgap_status mixed_once(pgap_engine *e, const uint32_t *words, const uint6
     }
     // bucket per number: the smallest digit count whose bound covers it
     constexpr int NB = MixedBuckets::n;
-    const int slack = kind == PGAP_FERMAT2 ? 4 : 14;
+    // headroom below R = 2^(28 digits): the variants' lazy-reduction bounds
+    const int slack = kind == PGAP_FERMAT2 ? 4 : 8 + (int)(select_bits(KIND_SPRP, 0, bs));
     std::vector<std::vector<uint32_t>> by_bucket(NB);
     size_t total_words = 0;
     for (size_t i = 0; i < count; ++i) {

@@ -116,7 +116,7 @@ __device__ __forceinline__ bool equal(const uint32_t (&a)[L], const uint32_t (&b
 }
 
 // Classifies x * R^{-1} mod N for a Montgomery-form x < R (every lazy bound
-// used here qualifies: x < 4N with N < R/16, x < 128N with N < R/2^14): 1 if
+// used here qualifies: x < 4N with N < R/16, x < 2bN with N < R/(4b^2)): 1 if
 // it is 1, 2 if it is N - 1, 0 otherwise.  This is the Montgomery reduction
 // of x itself, (x + mN)/R, which is congruent to x R^{-1} and lies in [0, N]
 // because x < R and m < R, so the two residues are recognised by their

@@ -1,18 +1,21 @@
 // mont_lg.h -- Montgomery arithmetic on numbers spread over a group of G
-// lanes (G = 4 or 8 lanes per number), radix 2^26.
+// lanes (G = 4 or 8 lanes per number), radix 2^28.
 //
 // Used where one lane cannot hold a number (above ~2240 bits the three
 // L-digit arrays of the one-lane kernel no longer fit the register file).
 // Lane g of a group owns digits [g*S, (g+1)*S) of x and N; L = G*S.
 //
 // Squaring is row-wise Montgomery (CIOS): for i = 0..L-1
-//     T += x_i * x;   m = T_0 * n' mod 2^26;   T += m * N;   T /= 2^26
+//     T += x_i * x;   m = T_0 * n' mod 2^28;   T += m * N;   T /= 2^28
 // m is broadcast inside the group with DPP (a modifier on a v_mov): quad_perm
 // within a quad, plus a row shift under a bank mask for groups of 8; the row
-// multipliers x_i are read from an LDS stage (see sqr_redc).  T is kept as 64-bit unnormalised digit accumulators; a
-// 26x26-bit product is < 2^52 and each accumulator takes two per row for at
-// most L = 168 rows, < 2^61, so no carry moves until the row loop ends.  The
-// division by 2^26 moves every accumulator down one position; positions are
+// multipliers x_i are read from an LDS stage (see sqr_redc).  T is kept as
+// 64-bit unnormalised digit accumulators.  A 28x28-bit product is < 2^56 and
+// an accumulator takes at most two per row while it moves from the top
+// position to the bottom, so up to 127 rows it stays below 2^64 with no carry
+// moved; wider numbers take one carry-save step halfway through the rows
+// (see carry_save).  The
+// division by 2^28 moves every accumulator down one position; positions are
 // mapped to registers with a rotating offset (position p of row i lives in
 // register (i + p) mod S), so the shift costs one 64-bit DPP move from the
 // next lane, not S register moves.  The S rows that multiply by one lane's
@@ -32,7 +35,7 @@
 namespace pgap {
 namespace lg {
 
-constexpr int RB = 26;
+constexpr int RB = 28;
 constexpr uint32_t MASK = (1u << RB) - 1;
 
 __device__ __forceinline__ uint64_t mad(uint32_t a, uint32_t b, uint64_t c) { return (uint64_t)a * b + c; }
@@ -218,11 +221,33 @@ __device__ __forceinline__ bool equal(const uint32_t (&a)[S], const uint32_t (&b
 constexpr int stage_stride(int S) { return S | 1; }
 constexpr size_t stage_bytes(int S) { return (size_t)256 * stage_stride(S) * sizeof(uint32_t); }
 
-// x <- x^2 R^{-1} mod N (lazy: x < 4N in, < 2N out), R = 2^(26*G*S).
+// Carry-save step at a block boundary of the row loop, where register p
+// holds position p: every accumulator keeps its low RB bits and passes the
+// rest up one position (across the lane boundary with one 64-bit DPP move).
+// The value of T is unchanged and every accumulator drops below 2^28 + 2^36.
+// The top lane's top position was refilled with zero by the last row, so no
+// carry leaves the window.
+template <int S, int G>
+__device__ __forceinline__ void carry_save(uint64_t (&t)[S], uint32_t g) {
+    uint64_t cin = gup64<G>(t[S - 1] >> RB);
+    if (g == 0) cin = 0;
+#pragma unroll
+    for (int p = S - 1; p >= 1; --p) t[p] = (t[p] & MASK) + (t[p - 1] >> RB);
+    t[0] = (t[0] & MASK) + cin;
+}
+
+// x <- x^2 R^{-1} mod N (lazy: x < 4N in, < 2N out), R = 2^(28*G*S).
 // `stage` is the group's LDS stage (G rows of stage_stride(S) words).
+//
+// Accumulator bound: a value enters at the top position as zero and takes two
+// products (< 2^56 each) per row for at most L rows, plus one carry (< 2^36)
+// at position 1: 2L * (2^28 - 1)^2 + 2^36 < 2^64 holds for L <= 128.  Above
+// that one carry-save after block G/2 - 1 splits the rows into two runs of
+// at most 127 (L <= 254).
 template <int S, int G>
 __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t np, uint32_t g,
                                          uint32_t *stage) {
+    static_assert(G * S <= 254 && (G * S <= 128 || (G / 2) * S <= 127), "accumulator bound");
     constexpr int ST = stage_stride(S);
     uint64_t t[S];
 #pragma unroll
@@ -247,6 +272,9 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S
             const uint64_t c = lane0 ? (t[u] >> RB) : 0ull;
             t[(u + 1) % S] += c;
             t[u] = top ? 0ull : in;
+        }
+        if constexpr (G * S > 128) {
+            if (o == G / 2 - 1) carry_save<S, G>(t, g);
         }
     }
     normalize<S, G>(t, x, g);

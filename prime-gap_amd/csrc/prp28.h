@@ -89,8 +89,9 @@ __device__ __forceinline__ void sqr_step(uint32_t (&x)[L], const uint32_t (&n)[L
 
 // x <- x * b for a small base b (b < 64): digits times b with the carry
 // propagated.  Lazy bound: x < 2N in gives x < 2bN out, and a squaring of an
-// input < 2bN returns < 2N as long as N < R / (4 b^2), which the strong-PRP
-// variants' max_bits (28L - 14) guarantees for every b < 64.
+// input < 2bN returns < 2N as long as N < R / (4 b^2): the strong-PRP
+// variants' max_bits (28L - 8) covers b <= 8, and the engine selects with six
+// more bits for larger bases (N < R/2^14 covers every b < 64).
 template <int L>
 __device__ __forceinline__ void mul_small(uint32_t (&x)[L], uint32_t b) {
     uint64_t c = 0;

@@ -1,5 +1,5 @@
 // prp_lg.h -- Fermat base 2 and strong-PRP tests with one number per group
-// of G lanes (radix 2^26, S digits per lane, L = G*S digits).  Same definitions
+// of G lanes (radix 2^28, S digits per lane, L = G*S digits).  Same definitions
 // and the same algorithm outline as the one-lane tests in prp28.h; every
 // multi-digit step runs distributed over the group (mont_lg.h).
 #pragma once
@@ -32,7 +32,7 @@ __device__ __forceinline__ uint32_t bitlen(const uint32_t (&d)[S], uint32_t g) {
     return gmax<G>(b);
 }
 
-// one = R mod N by modular doublings from 2^(b-1) (26*4S - (b-1) of them).
+// one = R mod N by modular doublings from 2^(b-1) (28*G*S - (b-1) of them).
 template <int S, int G>
 __device__ __forceinline__ void mont_one(const uint32_t (&n)[S], uint32_t b, uint32_t g, uint32_t (&one)[S]) {
     const uint32_t top = b - 1, td = top / RB, tb = top % RB;

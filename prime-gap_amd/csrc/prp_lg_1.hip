@@ -1,5 +1,5 @@
-// Instantiation unit: radix-2^26 variants, a quad (4 lanes) per number, S = 32 digits per lane.
+// Instantiation unit: radix-2^28 variants, a quad (4 lanes) per number, S = 30 digits per lane.
 #include "prp_lg_impl.h"
 namespace pgap {
-static RegisterLG<4, 32> registered_lg_1;
+static RegisterLG<4, 30> registered_lg_1;
 }

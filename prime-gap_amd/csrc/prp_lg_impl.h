@@ -1,4 +1,4 @@
-// prp_lg_impl.h -- batch kernels for the lane-group radix-2^26 variants (G = 4
+// prp_lg_impl.h -- batch kernels for the lane-group radix-2^28 variants (G = 4
 // or 8 lanes per number); included by the prp_lg_*.hip instantiation units.
 #pragma once
 #include <algorithm>
@@ -40,12 +40,10 @@ __global__ void __launch_bounds__(256) prp_lg_kernel(const uint32_t *__restrict_
     const uint32_t row = d_idx ? d_idx[item] : item;
     const uint32_t *nw = n + (size_t)row * limbs;
     uint32_t v;
-    if (KIND == KIND_FERMAT2) {
+    if (KIND == KIND_FERMAT2)
         v = lg::fermat2<S, G>(nw, limbs, g, stage);
-    } else {
-        v = 1;
-        for (uint32_t k = 0; k < bases.n && v; ++k) v = lg::sprp<S, G>(nw, limbs, bases.b[k], g, stage);
-    }
+    else
+        v = lg::sprp<S, G>(nw, limbs, bases.b[0], g, stage);
     if (g == 0) out[row] = (uint8_t)v;
 }
 
@@ -53,6 +51,7 @@ template <int S, int G, int KIND>
 hipError_t launch_lg(const uint32_t *d_n, uint32_t limbs, uint32_t count, const uint32_t *d_count,
                      const uint32_t *d_idx, Bases bases, uint8_t *d_out, int blocks_cap, hipStream_t stream) {
     if (count == 0) return hipSuccess;
+    if (KIND == KIND_SPRP && bases.n != 1) return hipErrorInvalidValue;
     const dim3 grid((uint32_t)(((size_t)count * G + 255) / 256));
     const size_t lds = lg_lds_bytes(prp_lg_kernel<S, G, KIND>, S, blocks_cap);
     hipLaunchKernelGGL((prp_lg_kernel<S, G, KIND>), grid, dim3(256), lds, stream, d_n, limbs, count, d_count, d_idx,
@@ -126,7 +125,7 @@ struct RegisterLG {
         (register_variant(Variant{lg::RB, G * Ss, G, KIND_FERMAT2, lg::RB * G * Ss - 4,
                                   &launch_lg<Ss, G, KIND_FERMAT2>, &occupancy_lg<Ss, G, KIND_FERMAT2>}),
          ...);
-        (register_variant(Variant{lg::RB, G * Ss, G, KIND_SPRP, lg::RB * G * Ss - 14, &launch_lg<Ss, G, KIND_SPRP>,
+        (register_variant(Variant{lg::RB, G * Ss, G, KIND_SPRP, lg::RB * G * Ss - 8, &launch_lg<Ss, G, KIND_SPRP>,
                                   &occupancy_lg<Ss, G, KIND_SPRP>, &launch_scan_lg<Ss, G>,
                                   &occupancy_scan_lg<Ss, G>}),
          ...);

@@ -50,12 +50,10 @@ prp_r28_kernel(const uint32_t *__restrict__ n, uint32_t limbs, uint32_t count, c
     const uint32_t row = d_idx ? d_idx[i] : i;
     const uint32_t *nw = n + (size_t)row * limbs;
     uint32_t v;
-    if (KIND == KIND_FERMAT2) {
+    if (KIND == KIND_FERMAT2)
         v = r28::fermat2<L>(nw, limbs, park);
-    } else {
-        v = 1;
-        for (uint32_t k = 0; k < bases.n && v; ++k) v = r28::sprp<L>(nw, limbs, bases.b[k], park);
-    }
+    else
+        v = r28::sprp<L>(nw, limbs, bases.b[0], park);
     out[row] = (uint8_t)v;
 }
 
@@ -63,6 +61,7 @@ template <int L, int KIND>
 hipError_t launch_r28(const uint32_t *d_n, uint32_t limbs, uint32_t count, const uint32_t *d_count,
                       const uint32_t *d_idx, Bases bases, uint8_t *d_out, int blocks_cap, hipStream_t stream) {
     if (count == 0) return hipSuccess;
+    if (KIND == KIND_SPRP && bases.n != 1) return hipErrorInvalidValue;
     const dim3 grid((count + 255u) / 256u);
     const size_t lds = r28_lds_bytes(prp_r28_kernel<L, KIND>,
                                      r28_park_bytes(L, KIND != KIND_FERMAT2 || r28::fermat_parks(L)), blocks_cap);
@@ -134,7 +133,7 @@ struct RegisterR28 {
         (register_variant(Variant{28, Ls, 1, KIND_FERMAT2, 28 * Ls - 4, &launch_r28<Ls, KIND_FERMAT2>,
                                   &occupancy_r28<Ls, KIND_FERMAT2>}),
          ...);
-        (register_variant(Variant{28, Ls, 1, KIND_SPRP, 28 * Ls - 14, &launch_r28<Ls, KIND_SPRP>,
+        (register_variant(Variant{28, Ls, 1, KIND_SPRP, 28 * Ls - 8, &launch_r28<Ls, KIND_SPRP>,
                                   &occupancy_r28<Ls, KIND_SPRP>, &launch_scan_r28<Ls>, &occupancy_scan_r28<Ls>}),
          ...);
     }

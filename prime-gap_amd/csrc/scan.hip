@@ -724,9 +724,13 @@ class Scanner {
         uint32_t bits = 0;
         for (uint32_t s : active) bits = std::max(bits, slots[s].bits);
         bits += 1;  // + x
+        Bases none{}, conf{};
+        static const uint32_t mr_bases[kMaxBases] = {3, 5, 7, 11, 13, 17, 19, 23};
+        conf.n = std::min<uint32_t>(confirm_rounds, kMaxBases);
+        for (uint32_t i = 0; i < conf.n; ++i) conf.b[i] = mr_bases[i];
         Choice cf, cc;
         if ((st = e->pick(KIND_FERMAT2, bits, &cf))) return st;
-        if (confirm_rounds && (st = e->pick(KIND_SPRP, bits, &cc))) return st;
+        if (confirm_rounds && (st = e->pick(KIND_SPRP, select_bits(KIND_SPRP, bits, conf), &cc))) return st;
         const Variant *vf = cf.v, *vc = cc.v;
         if (vc && !vc->scan) {
             e->set_error("scan: variant radix %d x %d has no scan kernel", vc->radix, vc->digits);
@@ -750,10 +754,6 @@ class Scanner {
         const uint32_t resident_full = (uint32_t)std::max(1, occ ? occ() : 1) * per_wave;
         const uint32_t resident = 2ull * na >= per_wave / 4 ? resident_full : std::min(resident_full, per_wave);
         const uint32_t cpad = 64u / (uint32_t)std::max(1, vr->lanes);  // items per wavefront
-        Bases none{}, conf{};
-        static const uint32_t mr_bases[kMaxBases] = {3, 5, 7, 11, 13, 17, 19, 23};
-        conf.n = std::min<uint32_t>(confirm_rounds, kMaxBases);
-        for (uint32_t i = 0; i < conf.n; ++i) conf.b[i] = mr_bases[i];
 
         memcpy(h_active, active.data(), na * 4);
         hipError_t he;

@@ -1,5 +1,5 @@
 // tune.hip -- the autotuner: for each number width, time every kernel family
-// (one lane per number in radix 2^28, a quad per number in radix 2^26) at its
+// (one lane per number, a group of 4 or of 8 lanes per number; radix 2^28) at its
 // smallest covering digit count and under a few occupancy caps, and keep the
 // fastest.  The table is plain text and is loaded by pgap_create through
 // pgap_options.tuning_path.
@@ -90,10 +90,9 @@ extern "C" pgap_status pgap_autotune(pgap_engine *e, const uint32_t *bits, size_
             TuneEntry best{};
             best.rate = -1;
             Bases bs{};
-            if (kind == KIND_SPRP) {
-                bs.n = 2;
+            if (kind == KIND_SPRP) {  // one base per launch (strong PRP to base 3)
+                bs.n = 1;
                 bs.b[0] = 3;
-                bs.b[1] = 5;
             }
             for (const Variant *v : cand) {
                 const int natural = v->blocks_per_cu ? v->blocks_per_cu() : 1;

@@ -42,6 +42,30 @@ def check(engine, vals, kind=pgap.FERMAT2, bases=()):
     return got
 
 
+@pytest.mark.parametrize("digits", [40, 78])
+def test_sprp_headroom_by_base(digits):
+    # Strong PRP multiplies by the base as a scalar: bases up to 8 run with
+    # N < R/2^8, larger ones need N < R/2^14.  Numbers of exactly 28*digits - 8
+    # bits (largest digits the lazy bound admits among them) run on `digits`
+    # digits with bases (3, 5, 7) and on a wider variant with base 61; both
+    # match the oracle.
+    bits = 28 * digits - 8
+    rng = random.Random(digits)
+    vals = [rng.getrandbits(bits) | (1 << (bits - 1)) | 1 for _ in range(126)]
+    vals += [(1 << bits) - 1, (1 << bits) - 3]
+    x = (1 << bits) - 1
+    while not ol.probab_prime(x):
+        x -= 2
+    vals.append(x)
+    with pgap.Engine(0) as e:   # untuned: the smallest covering variant
+        got = check(e, vals, pgap.SPRP, (3, 5, 7))
+        assert got[-1] == 1
+        assert e.last_launch().digits == digits
+        got = check(e, vals, pgap.SPRP, (3, 61))
+        assert got[-1] == 1
+        assert e.last_launch().digits > digits
+
+
 def test_special_values_fermat(engine):
     vals = special_values()
     for v in vals:
@@ -96,8 +120,8 @@ def test_large_family_and_sprp(engine):
 
 @pytest.mark.parametrize("bits", [2300, 2490, 2600, 3000, 3320, 3500, 4100, 4250])
 def test_quad_variants_random_sizes(engine, bits):
-    # wider than one lane holds: four lanes per number up to 3324 bits, eight
-    # above (the default choice where each measured faster)
+    # wider than one lane holds: four lanes per number up to 3356 bits (radix
+    # 2^28, 120 digits), eight above
     rng = random.Random(bits)
     vals = [rng.getrandbits(bits) | (1 << (bits - 1)) | 1 for _ in range(96)]
     p = []
@@ -108,7 +132,7 @@ def test_quad_variants_random_sizes(engine, bits):
         p.append(x)
     got = check(engine, vals + p)
     assert got[-2:].all()
-    assert engine.last_launch().lanes_per_number == (4 if bits <= 3324 else 8)
+    assert engine.last_launch().lanes_per_number == (4 if bits <= 28 * 120 - 4 else 8)
 
 
 @pytest.fixture(scope="module")
@@ -120,7 +144,7 @@ def engine8():
 
 @pytest.mark.parametrize("bits", [1500, 2128, 2280, 2300, 3000, 3320, 3500, 4100, 4250, 4300, 4360])
 def test_octet_variants_random_sizes(engine8, bits):
-    # eight lanes per number (radix 2^26, 21 digits per lane): up to 4364 bits
+    # eight lanes per number (radix 2^28, up to 20 digits per lane): up to 4476 bits
     rng = random.Random(bits + 8)
     vals = [rng.getrandbits(bits) | (1 << (bits - 1)) | 1 for _ in range(64)]
     x = vals[0]
@@ -129,7 +153,7 @@ def test_octet_variants_random_sizes(engine8, bits):
     got = check(engine8, vals + [x])
     assert got[-1] == 1
     assert engine8.last_launch().lanes_per_number == 8
-    if bits <= 26 * 168 - 14:
+    if bits <= 28 * 160 - 14:
         check(engine8, vals[:24] + [x], pgap.SPRP, (3, 5))
         assert engine8.last_launch().lanes_per_number == 8
 
@@ -149,7 +173,7 @@ def test_quad_variants_special_and_sprp(engine):
             (1 << 3001) - 1]
     got = check(engine, vals)
     assert list(got[:4]) == [1, 1, 1, 1]
-    strong = [v for v in vals if v.bit_length() <= 26 * 168 - 14]  # widest strong-PRP variant
+    strong = [v for v in vals if v.bit_length() <= 28 * 160 - 14]  # widest strong-PRP variant
     check(engine, strong, pgap.SPRP, (2,))
     check(engine, strong, pgap.SPRP, (3, 5))
     c = candidates(3001, 2190, 1, 100_000, 40000, 64, 17)   # 4231-4248-bit family
