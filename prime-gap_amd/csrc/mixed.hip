@@ -65,6 +65,10 @@ static pgap_status mixed_once(pgap_engine *e, const uint32_t *words, const uint6
             }
             bs.b[i] = bases[i];
         }
+        if (nbases > 1) {  // the kernel tests bases.b[0]; pgap_prp_batch_mixed stages the others
+            e->set_error("pgap_prp_batch_mixed: one base per persistent launch");
+            return PGAP_E_ARG;
+        }
     } else if (kind != PGAP_FERMAT2) {
         e->set_error("unknown test kind %d", (int)kind);
         return PGAP_E_ARG;

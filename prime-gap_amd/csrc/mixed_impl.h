@@ -28,15 +28,16 @@ using MixedBuckets = Buckets<12, 26, 40, 56, 78>;
 constexpr size_t kMixedParkBytes =
     (size_t)MixedBuckets::digits[MixedBuckets::n - 1] * r28::PARK_STRIDE * sizeof(uint32_t);
 
+// One test of one number.  A strong-PRP launch tests one base, bases.b[0]
+// (the host runs later bases on the survivors), so no loop keeps a test live
+// across bases.  The kernel is scheduled for two wavefronts per SIMD, the
+// batch kernels' choice for the widest bucket, so the wide squaring uses one
+// chain per column.
 template <int KIND, int L>
 __device__ __forceinline__ uint32_t test_one(const uint32_t *nw, uint32_t limbs, const Bases &bases,
                                              uint32_t *park) {
-    // the persistent kernel runs one wavefront per SIMD (its widest bucket
-    // sets the registers): three chains per column in the wide squaring
-    if (KIND == KIND_FERMAT2) return r28::fermat2<L, 3>(nw, limbs, park);
-    uint32_t v = 1;
-    for (uint32_t k = 0; k < bases.n && v; ++k) v = r28::sprp<L, 3>(nw, limbs, bases.b[k], park);
-    return v;
+    if (KIND == KIND_FERMAT2) return r28::fermat2<L, 1>(nw, limbs, park);
+    return r28::sprp<L, 1>(nw, limbs, bases.b[0], park);
 }
 
 template <int KIND, int... Ls>
@@ -49,7 +50,7 @@ __device__ __forceinline__ uint32_t dispatch(int bucket, const uint32_t *nw, uin
 }
 
 template <int KIND>
-__global__ void __launch_bounds__(256) mixed_kernel(const uint32_t *__restrict__ words,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) mixed_kernel(const uint32_t *__restrict__ words,
                                                     const uint64_t *__restrict__ offset,
                                                     const uint32_t *__restrict__ limbs,
                                                     const uint32_t *__restrict__ order,
@@ -89,18 +90,31 @@ struct MixedLaunch {
 hipError_t launch_mixed_fermat(const MixedLaunch &a, int device, hipStream_t stream);
 hipError_t launch_mixed_sprp(const MixedLaunch &a, int device, hipStream_t stream);
 
+// The kernel is scheduled for two wavefronts per SIMD, which pays when every
+// SIMD works through several tiles (2^19 numbers, 8192 tiles: 160.8 ms against
+// 182.4 ms for the one-wave build).  With few tiles the dispatcher may place
+// the first (widest) tiles two to a SIMD while other SIMDs idle (163840
+// numbers, 2560 tiles: 86.5 ms against 53.1 ms), so below two tiles per
+// wave slot at two waves the launch reserves enough LDS to hold one workgroup
+// (one wave per SIMD) per CU.
+constexpr uint32_t kMixedTilesPerCuTwoWaves = 16;
+constexpr size_t kMixedOneWaveLds = 96 * 1024;  // > 160 KiB / 2: one workgroup per CU
+
 template <int KIND>
 hipError_t launch_mixed(const MixedLaunch &a, int device, hipStream_t stream) {
     int per_cu = 0, cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+        cus = 256;
+    const bool two_waves = a.ntiles >= (uint32_t)cus * kMixedTilesPerCuTwoWaves;
+    const size_t lds = two_waves ? kMixedParkBytes : std::max(kMixedParkBytes, kMixedOneWaveLds);
     (void)hipFuncSetAttribute((const void *)mixed_kernel<KIND>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kMixedParkBytes);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mixed_kernel<KIND>, 256, kMixedParkBytes) != hipSuccess ||
+                              (int)lds);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mixed_kernel<KIND>, 256, lds) != hipSuccess ||
         per_cu <= 0)
         per_cu = 1;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 256;
     const uint32_t grid = (uint32_t)std::min<size_t>((size_t)per_cu * cus, ((size_t)a.ntiles + 3) / 4);
-    hipLaunchKernelGGL(mixed_kernel<KIND>, dim3(std::max(grid, 1u)), dim3(256), kMixedParkBytes, stream, a.words, a.offset, a.limbs,
-                       a.order, a.tile_bucket, a.ntiles, a.next_tile, a.bases, a.out);
+    hipLaunchKernelGGL(mixed_kernel<KIND>, dim3(std::max(grid, 1u)), dim3(256), lds, stream, a.words, a.offset,
+                       a.limbs, a.order, a.tile_bucket, a.ntiles, a.next_tile, a.bases, a.out);
     return hipGetLastError();
 }
 
