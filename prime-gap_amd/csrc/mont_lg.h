@@ -10,16 +10,14 @@
 // m is broadcast inside the group with DPP (a modifier on a v_mov): quad_perm
 // within a quad, plus a row shift under a bank mask for groups of 8; the row
 // multipliers x_i are read from an LDS stage (see sqr_redc).  T is kept as
-// 64-bit unnormalised digit accumulators.  A 28x28-bit product is < 2^56 and
-// an accumulator takes at most two per row while it moves from the top
-// position to the bottom, so up to 127 rows it stays below 2^64 with no carry
-// moved; wider numbers take one carry-save step halfway through the rows
-// (see carry_save).  The
-// division by 2^28 moves every accumulator down one position; positions are
-// mapped to registers with a rotating offset (position p of row i lives in
-// register (i + p) mod S), so the shift costs one 64-bit DPP move from the
-// next lane, not S register moves.  The S rows that multiply by one lane's
-// digits share one unrolled body.
+// 64-bit unnormalised digit accumulators.  The division by 2^28 moves every
+// accumulator down one position; positions are mapped to registers with a
+// rotating offset (position p of row i lives in register (i + p) mod S), so
+// the shift moves only each lane's bottom accumulator to the lane below, not
+// S registers.  Before it moves, every lane adds that accumulator's bits
+// above 28 to its own next position, so what crosses the lane boundary is a
+// 28-bit digit: one 32-bit DPP move per row (see sqr_redc for the bound).
+// The S rows that multiply by one lane's digits share one unrolled body.
 //
 // G = 8 halves the registers per lane (5S + temporaries), which lets a
 // 4247-bit number run at more than one wavefront per SIMD without the
@@ -68,21 +66,11 @@ __device__ __forceinline__ uint32_t gbcast_top(uint32_t v) {
     if constexpr (G == 4) return q;
     return (uint32_t)__builtin_amdgcn_update_dpp((int)q, (int)q, DPP_ROW_SHL4, 0xF, 0x5, false);
 }
-// lane g <- lane g+1 (the top lane's value is undefined; callers mask it)
-template <int G>
-__device__ __forceinline__ uint32_t gdown(uint32_t v) {
-    if constexpr (G == 4) return qperm<1, 2, 3, 3>(v);
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_ROW_SHL1, 0xF, 0xF, false);
-}
 // lane g <- lane g-1 (lane 0's value is undefined; callers mask it)
 template <int G>
 __device__ __forceinline__ uint32_t gup(uint32_t v) {
     if constexpr (G == 4) return qperm<0, 0, 1, 2>(v);
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_ROW_SHR1, 0xF, 0xF, false);
-}
-template <int G>
-__device__ __forceinline__ uint64_t gdown64(uint64_t v) {
-    return (uint64_t)gdown<G>((uint32_t)v) | ((uint64_t)gdown<G>((uint32_t)(v >> 32)) << 32);
 }
 template <int G>
 __device__ __forceinline__ uint64_t gup64(uint64_t v) {
@@ -221,33 +209,28 @@ __device__ __forceinline__ bool equal(const uint32_t (&a)[S], const uint32_t (&b
 constexpr int stage_stride(int S) { return S | 1; }
 constexpr size_t stage_bytes(int S) { return (size_t)256 * stage_stride(S) * sizeof(uint32_t); }
 
-// Carry-save step at a block boundary of the row loop, where register p
-// holds position p: every accumulator keeps its low RB bits and passes the
-// rest up one position (across the lane boundary with one 64-bit DPP move).
-// The value of T is unchanged and every accumulator drops below 2^28 + 2^36.
-// The top lane's top position was refilled with zero by the last row, so no
-// carry leaves the window.
-template <int S, int G>
-__device__ __forceinline__ void carry_save(uint64_t (&t)[S], uint32_t g) {
-    uint64_t cin = gup64<G>(t[S - 1] >> RB);
-    if (g == 0) cin = 0;
-#pragma unroll
-    for (int p = S - 1; p >= 1; --p) t[p] = (t[p] & MASK) + (t[p - 1] >> RB);
-    t[0] = (t[0] & MASK) + cin;
+// lane g <- lane g+1 across the whole 16-lane DPP row (the row's last lane
+// reads zero); a group's top lane reads the next group's lane 0, which the
+// caller replaces.
+__device__ __forceinline__ uint32_t row_down(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, DPP_ROW_SHL1, 0xF, 0xF, true);
 }
 
 // x <- x^2 R^{-1} mod N (lazy: x < 4N in, < 2N out), R = 2^(28*G*S).
 // `stage` is the group's LDS stage (G rows of stage_stride(S) words).
 //
-// Accumulator bound: a value enters at the top position as zero and takes two
-// products (< 2^56 each) per row for at most L rows, plus one carry (< 2^36)
-// at position 1: 2L * (2^28 - 1)^2 + 2^36 < 2^64 holds for L <= 128.  Above
-// that one carry-save after block G/2 - 1 splits the rows into two runs of
-// at most 127 (L <= 254).
+// One row: T += x_i * x; m from the bottom digit (lane 0); T += m * N; then
+// every lane adds its bottom accumulator's bits above 28 to its next
+// position and hands the low 28 bits to the lane below; the group's top lane
+// takes zero.  Lane 0's low bits are zero (that is what m is for), so
+// T /= 2^28 is exact.  Accumulator bound: a value enters a lane below 2^28,
+// takes two products (< 2^56 each) per row for at most S rows and one carry
+// below 2^36, and leaves the lane at 28 bits again: 2S(2^28 - 1)^2 + 2^36 +
+// 2^28 < 2^64 for S <= 127, whatever G.
 template <int S, int G>
 __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t np, uint32_t g,
                                          uint32_t *stage) {
-    static_assert(G * S <= 254 && (G * S <= 128 || (G / 2) * S <= 127), "accumulator bound");
+    static_assert(S <= 127 && (G == 4 || G == 8), "accumulator bound; groups tile a 16-lane DPP row");
     constexpr int ST = stage_stride(S);
     uint64_t t[S];
 #pragma unroll
@@ -255,7 +238,7 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S
         t[p] = 0;
         stage[g * ST + p] = x[p];
     }
-    const bool lane0 = (g == 0), top = (g == G - 1);
+    const bool top = (g == G - 1);
     for (int o = 0; o < G; ++o) {
         uint32_t xb[S];
 #pragma unroll
@@ -268,13 +251,9 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S
             const uint32_t m = gbcast0<G>(((uint32_t)t[u] * np) & MASK);
 #pragma unroll
             for (int j = 0; j < S; ++j) t[(u + j) % S] = mad(m, n[j], t[(u + j) % S]);
-            const uint64_t in = gdown64<G>(t[u]);
-            const uint64_t c = lane0 ? (t[u] >> RB) : 0ull;
-            t[(u + 1) % S] += c;
-            t[u] = top ? 0ull : in;
-        }
-        if constexpr (G * S > 128) {
-            if (o == G / 2 - 1) carry_save<S, G>(t, g);
+            t[(u + 1) % S] += t[u] >> RB;
+            const uint32_t lo = row_down((uint32_t)t[u] & MASK);
+            t[u] = top ? 0u : lo;
         }
     }
     normalize<S, G>(t, x, g);
