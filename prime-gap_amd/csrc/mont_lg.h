@@ -216,8 +216,48 @@ __device__ __forceinline__ uint32_t row_down(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, DPP_ROW_SHL1, 0xF, 0xF, true);
 }
 
+// Digits of a lane-local accumulator array t times 2^s (s in {0,1}) with one
+// cross-lane step instead of G - 1: each lane carries through its own
+// positions, doubles its (now exact) digits, and adds the carry out of the
+// lane below, split into two digits, to its two bottom digits.  Digits 0 and
+// 1 of a lane may then exceed 28 bits (< 2^29 and < 2^28 + 2^10); every
+// other digit is exact.  The value is exact: the top lane's own carry is zero
+// whenever the value is below R.  sqr_redc accepts such digits (see its
+// bound); everything else wants canon() first.
+template <int S, int G>
+__device__ __forceinline__ void lazy_carry(const uint64_t (&t)[S], uint32_t (&x)[S], uint32_t g, uint32_t s) {
+    uint64_t c = 0;
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+        const uint64_t v = t[p] + c;
+        x[p] = (uint32_t)v & MASK;
+        c = v >> RB;
+    }
+    const uint32_t out = (x[S - 1] >> (RB - 1)) & s;  // the doubling's bit out of the top digit
+#pragma unroll
+    for (int p = S - 1; p >= 1; --p) x[p] = ((x[p] << s) | ((x[p - 1] >> (RB - 1)) & s)) & MASK;
+    x[0] = (x[0] << s) & MASK;
+    c = (c << s) + out;  // the lane's carry out, < 2^38
+    uint32_t c0 = gup<G>((uint32_t)c & MASK), c1 = gup<G>((uint32_t)(c >> RB));
+    if (g == 0) c0 = c1 = 0;
+    x[0] += c0;
+    x[1] += c1;
+}
+
+// Exact 28-bit digits from lazy ones (lazy_carry's output or any digits whose
+// value is below R).
+template <int S, int G>
+__device__ __forceinline__ void canon(uint32_t (&x)[S], uint32_t g) {
+    uint64_t t[S];
+#pragma unroll
+    for (int p = 0; p < S; ++p) t[p] = x[p];
+    normalize<S, G>(t, x, g);
+}
+
 // x <- x^2 R^{-1} mod N (lazy: x < 4N in, < 2N out), R = 2^(28*G*S).
 // `stage` is the group's LDS stage (G rows of stage_stride(S) words).
+// LAZY: the result comes as lazy_carry digits times 2^s (< 4N); otherwise as
+// exact digits (s must be 0).
 //
 // One row: T += x_i * x; m from the bottom digit (lane 0); T += m * N; then
 // every lane adds its bottom accumulator's bits above 28 to its next
@@ -226,11 +266,13 @@ __device__ __forceinline__ uint32_t row_down(uint32_t v) {
 // T /= 2^28 is exact.  Accumulator bound: a value enters a lane below 2^28,
 // takes two products (< 2^56 each) per row for at most S rows and one carry
 // below 2^36, and leaves the lane at 28 bits again: 2S(2^28 - 1)^2 + 2^36 +
-// 2^28 < 2^64 for S <= 127, whatever G.
-template <int S, int G>
+// 2^28 < 2^64 for S <= 127, whatever G, with exact input digits.  With
+// lazy_carry digits (each product < 2^58) the same count gives
+// S(2^58 + 2^56) + 2^37 < 2^64 for S <= 32.
+template <int S, int G, bool LAZY = false>
 __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t np, uint32_t g,
-                                         uint32_t *stage) {
-    static_assert(S <= 127 && (G == 4 || G == 8), "accumulator bound; groups tile a 16-lane DPP row");
+                                         uint32_t *stage, uint32_t s = 0) {
+    static_assert(S <= 32 && (G == 4 || G == 8), "accumulator bound; groups tile a 16-lane DPP row");
     constexpr int ST = stage_stride(S);
     uint64_t t[S];
 #pragma unroll
@@ -256,7 +298,10 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S
             t[u] = top ? 0u : lo;
         }
     }
-    normalize<S, G>(t, x, g);
+    if constexpr (LAZY)
+        lazy_carry<S, G>(t, x, g, s);
+    else
+        normalize<S, G>(t, x, g);
 }
 
 }  // namespace lg

@@ -72,11 +72,11 @@ __device__ __forceinline__ uint32_t fermat2(const uint32_t *__restrict__ nw, uin
         const uint32_t s = (i == 0) ? 0u : ((ew >> (i & 31)) & 1u);
         uint32_t nxt = ew;
         if ((i & 31) == 0 && i > 0) nxt = nw[(i >> 5) - 1];
-        sqr_redc<S, G>(x, n, np, g, stage);
-        shl<S, G>(x, s, g);
+        sqr_redc<S, G, true>(x, n, np, g, stage, s);
         ew = nxt;
         --i;
     }
+    canon<S, G>(x, g);
     cond_sub<S, G>(x, n, g);
     uint32_t one[S];
     mont_one<S, G>(n, b, g, one);
@@ -139,12 +139,12 @@ __device__ __forceinline__ uint32_t sprp(const uint32_t *__restrict__ nw, uint32
         const uint32_t bit = (ew >> (i & 31)) & 1u;
         uint32_t nxt = ew;
         if ((i & 31) == 0 && i > tz) nxt = nw[(i >> 5) - 1];
-        sqr_redc<S, G>(x, n, np, g, stage);
-        shl<S, G>(x, bb == 2 ? bit : 0u, g);
-        if (bb != 2 && bit) mul_small<S, G>(x, bb, g);
+        sqr_redc<S, G, true>(x, n, np, g, stage, bb == 2 ? bit : 0u);
+        if (bb != 2 && bit) mul_small<S, G>(x, bb, g);  // exact digits out
         ew = nxt;
         --i;
     }
+    canon<S, G>(x, g);
     // Phase 2: x = Mont(b^d) < 2*bb*N; reduce, compare with Mont(+-1), and
     // square at most tz - 1 times looking for Mont(-1)
     for (int r = 0; r < 128; ++r) {
