@@ -344,7 +344,20 @@ struct SlotHost {
     bool fb[2] = {false, false};
     double t_submit = 0;
     uint32_t bits = 0;             // bit length of the centre
+    uint64_t room = 0;             // 2^bits - centre, saturated at 2^40
 };
+
+// 2^bitlen(c) - c, saturated at 2^40: next-side candidates centre + x with
+// x < room keep the centre's bit length.
+static uint64_t room_above(const Words &c) {
+    constexpr uint64_t kCap = 1ull << 40;
+    Words d = c;
+    add_signed(d, (int64_t)kCap);
+    const uint32_t b = bitlen(c);
+    if (bitlen(d) == b) return kCap;
+    const uint64_t lo = (uint64_t)(c.size() > 0 ? c[0] : 0) | ((uint64_t)(c.size() > 1 ? c[1] : 0) << 32);
+    return b >= 64 ? 0 - lo : (1ull << b) - lo;  // < 2^40: the low 64 bits decide it
+}
 
 struct Staged {
     FamInfo *fam;
@@ -672,6 +685,7 @@ class Scanner {
             sh.fb_from[0] = sh.fb_from[1] = f.SL + 1;
             sh.t_submit = r.t_submit;
             sh.bits = bitlen(c);
+            sh.room = room_above(c);
             Upload d{};
             d.slot = slot;
             d.kind = 0;
@@ -721,9 +735,15 @@ class Scanner {
         if ((st = admit())) return st;
         if (active.empty()) return PGAP_OK;
         const uint32_t na = (uint32_t)active.size();
+        // widest N this round: a next-side candidate centre + x gains a bit
+        // only when x reaches the room above the centre (x stays below the
+        // sieve length or the fallback screen's next offset)
         uint32_t bits = 0;
-        for (uint32_t s : active) bits = std::max(bits, slots[s].bits);
-        bits += 1;  // + x
+        for (uint32_t s : active) {
+            const SlotHost &sh = slots[s];
+            const uint64_t xmax = std::max<uint64_t>(sh.fam->SL, std::max(sh.fb_from[0], sh.fb_from[1]));
+            bits = std::max(bits, sh.bits + (xmax >= sh.room ? 1u : 0u));
+        }
         Bases none{}, conf{};
         static const uint32_t mr_bases[kMaxBases] = {3, 5, 7, 11, 13, 17, 19, 23};
         conf.n = std::min<uint32_t>(confirm_rounds, kMaxBases);
