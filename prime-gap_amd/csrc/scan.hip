@@ -52,6 +52,15 @@ constexpr uint32_t FALLBACK_CHUNK = 4096;  // candidates appended per exhausted 
 constexpr uint32_t FALLBACK_SPAN = 65536;  // offsets screened per append
 constexpr uint32_t MAX_SLOTS = 1u << 16;  // m's in flight (the HBM budget may allow fewer)
 
+// Latency-first rounds.  A round costs one test's latency whatever it
+// carries, and a group of 8 lanes finishes a wide test sooner than one lane
+// (2128 bits, at one wave per SIMD: 18.6 against 44.5 ms) at under half the
+// throughput.  While few sides scan, the group's one-wave residency (an
+// eighth of the one-lane count) still gives each side a deep window, so such
+// rounds run on the 8-lane variants when the one-lane variant is wide.
+constexpr uint32_t LATENCY_SIDES = 128;      // scanning sides at most
+constexpr int LATENCY_MIN_DIGITS = 52;       // one-lane variant at least this wide
+
 enum : uint32_t {
     F_PREV_DONE = 1,
     F_NEXT_DONE = 2,
@@ -749,8 +758,18 @@ class Scanner {
         conf.n = std::min<uint32_t>(confirm_rounds, kMaxBases);
         for (uint32_t i = 0; i < conf.n; ++i) conf.b[i] = mr_bases[i];
         Choice cf, cc;
+        const uint32_t sbits = select_bits(KIND_SPRP, bits, conf);
         if ((st = e->pick(KIND_FERMAT2, bits, &cf))) return st;
-        if (confirm_rounds && (st = e->pick(KIND_SPRP, select_bits(KIND_SPRP, bits, conf), &cc))) return st;
+        if (confirm_rounds && (st = e->pick(KIND_SPRP, sbits, &cc))) return st;
+        if (2 * na <= LATENCY_SIDES && !e->opt.lanes_per_number && !e->opt.radix && cf.v->lanes == 1 &&
+            cf.v->digits >= LATENCY_MIN_DIGITS) {
+            const Variant *lf = find_variant(KIND_FERMAT2, (int)bits, 28, 8);
+            const Variant *lc = confirm_rounds ? find_variant(KIND_SPRP, (int)sbits, 28, 8) : nullptr;
+            if (lf && (!confirm_rounds || (lc && lc->scan))) {
+                cf = Choice{lf, 0};
+                cc = Choice{lc, 0};
+            }
+        }
         const Variant *vf = cf.v, *vc = cc.v;
         if (vc && !vc->scan) {
             e->set_error("scan: variant radix %d x %d has no scan kernel", vc->radix, vc->digits);

@@ -100,6 +100,30 @@ def test_confirmation_rejects_fermat_pseudoprimes(confirm):
         (want["prev_p"], want["next_p"], want["p_tests"], want["n_tests"], 0)
 
 
+@pytest.mark.parametrize("confirm", [0, 2])
+def test_width_grows_past_a_power_of_two(confirm):
+    # K = 2 and centres 2m just below 2^48: the next-side candidates 2m + x
+    # reach 49 bits (the next primes of the first two m's are 2^48 + 21), one
+    # bit above the widest N the 2-digit strong-PRP variant registers
+    # (28*2 - 8).  A round's width counts the room above each centre
+    # (20, 6 and 1400 here), so these rounds run on a wider variant; rows
+    # match the oracle.
+    P, D, SL = 11, 1155, 2000
+    ms = [2**47 - 10, 2**47 - 3, 2**47 - 700]
+    sv = ol.Sieve(P, D, SL, 10000)
+    lists = {m: sv(m) for m in ms}
+    kind = 0 if confirm else 1
+    want = {m: ol.test_interval(P, D, m, SL, *lists[m], kind=kind) for m in ms}
+    assert (2 * ms[0] + want[ms[0]]["next_p"]).bit_length() == 49
+    with pgap.Engine(0) as e:
+        e.scan_config(64, confirm)
+        fam = e.family(P, D, SL)
+        rows = run_scan(e, fam, [(m, *lists[m]) for m in ms])
+    for m in ms:
+        r, w = rows[m], want[m]
+        assert (r.prev_p, r.next_p, r.p_tests, r.n_tests) == (w["prev_p"], w["next_p"], w["p_tests"], w["n_tests"]), m
+
+
 @pytest.mark.parametrize("min_merit", [6.0, 7.5])
 def test_side_skip_follows_gap_test_gpu_rule(unknowns_907, min_merit):
     # gap_test_gpu.cu:591-617: with the prev PRP found and the next side not
