@@ -290,6 +290,18 @@ def bench_stream(engine, pgap, n_m, load, seed, cpu_sample, threads, P4=1511, D4
     gaps = np.cumsum(rng.exponential(1.0 / (load * cap_mps), size=len(lat_set)))
     rows_lat, lat_s = run(lat_set, gaps)
     lat_ms = np.array([lat for _, lat in rows_lat.values()]) * 1e3
+    # one m at a time (no other work in flight): the lightly loaded latency
+    single = []
+    for i in cap_set[:64]:
+        p, q = lists[i]
+        t_sub = time.monotonic()
+        assert engine.scan_submit(fam, int(order[i]), p, q, tag=i) == 0
+        got = []
+        while not got:
+            got = engine.scan_poll(wait=True)
+        single.append((got[0].t_done - t_sub) * 1e3)
+        r0 = rows_cap[i][0]
+        assert (got[0].prev_p, got[0].next_p) == (r0.prev_p, r0.next_p)
     out = {
         "workload": f"P={P4} d={D4} SL={SL4}, m in seeded random order from [1,{m_hi}) coprime to d, unknowns "
                     f"sieved to {sieve_limit:.0e} ({np.mean([len(p) + len(q) for p, q in lists]):.0f} per m), Fermat-2 scan "
@@ -302,6 +314,8 @@ def bench_stream(engine, pgap, n_m, load, seed, cpu_sample, threads, P4=1511, D4
         "offered_load": load,
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
+        "single_m_p50_ms": float(np.percentile(single, 50)),
+        "single_m_p99_ms": float(np.percentile(single, 99)),
         "latency_phase_s": lat_s,
         "sieve_s_untimed": sieve_s,
         "slots": s1["slots"],
