@@ -83,13 +83,17 @@ __device__ __forceinline__ uint32_t fermat2(const uint32_t *__restrict__ nw, uin
     return equal<S, G>(x, one) ? 1u : 0u;
 }
 
-// x <- x * b (b < 64), digits renormalised.
-template <int S, int G>
+// x <- x * b (b < 64).  LAZY: lazy_carry digits out (the exponent loop's
+// form; lazy digits in, each product < 2^35); otherwise exact digits.
+template <int S, int G, bool LAZY = false>
 __device__ __forceinline__ void mul_small(uint32_t (&x)[S], uint32_t b, uint32_t g) {
     uint64_t t[S];
 #pragma unroll
     for (int p = 0; p < S; ++p) t[p] = (uint64_t)x[p] * b;
-    normalize<S, G>(t, x, g);
+    if constexpr (LAZY)
+        lazy_carry<S, G>(t, x, g, 0u);
+    else
+        normalize<S, G>(t, x, g);
 }
 
 // a <- (a + c) mod N for a, c < N.
@@ -140,7 +144,7 @@ __device__ __forceinline__ uint32_t sprp(const uint32_t *__restrict__ nw, uint32
         uint32_t nxt = ew;
         if ((i & 31) == 0 && i > tz) nxt = nw[(i >> 5) - 1];
         sqr_redc<S, G, true>(x, n, np, g, stage, bb == 2 ? bit : 0u);
-        if (bb != 2 && bit) mul_small<S, G>(x, bb, g);  // exact digits out
+        if (bb != 2 && bit) mul_small<S, G, true>(x, bb, g);
         ew = nxt;
         --i;
     }
