@@ -51,7 +51,7 @@ typedef struct {
     int device;               /* HIP device ordinal (the reference hard-codes 0, miller_rabin.h:326) */
     size_t hbm_budget;        /* bytes the streaming scan may keep resident; 0 = 1/4 of free HBM */
     const char *tuning_path;  /* autotuner table written by pgap_autotune(); NULL = built-in defaults */
-    int radix;                /* force 28 or 32 (0 = tuned) */
+    int radix;                /* force the digit radix in bits (every variant is 28; 0 = tuned) */
     int lanes_per_number;     /* force lanes per number (0 = tuned) */
     int waves_per_simd;       /* force an occupancy cap: resident 256-thread workgroups per CU, i.e.
                                  waves per SIMD (0 = tuned, else the kernel's natural occupancy) */
@@ -61,6 +61,8 @@ typedef struct {
  * Mirrors test_runner_t's constructor/destructor (miller_rabin.h:317-339). */
 pgap_status pgap_create(const pgap_options *opt, pgap_engine **out);
 void pgap_destroy(pgap_engine *e);
+/* Message of the engine's last failure, copied for the calling thread: valid
+ * until that thread calls pgap_last_error again. */
 const char *pgap_last_error(const pgap_engine *e);
 /* Library version string, e.g. "pgap 0.1 gfx950". */
 const char *pgap_version(void);
@@ -94,7 +96,7 @@ pgap_status pgap_sync(pgap_engine *e);
 typedef struct {
     double kernel_ms;
     uint32_t digits;            /* digits per number of the variant */
-    uint32_t radix_bits;        /* 28 or 32 */
+    uint32_t radix_bits;        /* digit radix in bits (28) */
     uint32_t lanes_per_number;
     uint32_t launches;
     uint64_t tests;

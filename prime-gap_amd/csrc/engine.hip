@@ -364,10 +364,16 @@ void pgap_destroy(pgap_engine *e) {
     delete e;
 }
 
+// The message is copied per calling thread: the scan worker may set a new
+// one while the caller reads this one.  Valid until this thread's next call.
 const char *pgap_last_error(const pgap_engine *e) {
     if (!e) return "null engine";
-    std::lock_guard<std::mutex> g(e->err_mu);
-    return e->err.c_str();
+    static thread_local std::string copy;
+    {
+        std::lock_guard<std::mutex> g(e->err_mu);
+        copy = e->err;
+    }
+    return copy.c_str();
 }
 
 pgap_status pgap_prp_batch(pgap_engine *e, uint32_t limbs, const uint32_t *n, size_t count, pgap_test kind,

@@ -59,7 +59,7 @@ typedef hipError_t (*scan_launch_fn)(const uint32_t *d_n, uint32_t limbs, uint32
                                      uint8_t *d_out, int blocks_cap, hipStream_t stream);
 
 struct Variant {
-    int radix;       // 28 or 32
+    int radix;       // digit radix in bits (28)
     int digits;      // digits (radix-sized) per number
     int lanes;       // lanes cooperating on one number
     int kind;        // Kind
