@@ -54,10 +54,13 @@ constexpr uint32_t MAX_SLOTS = 1u << 16;  // m's in flight (the HBM budget may a
 
 // Latency-first rounds.  A round costs one test's latency whatever it
 // carries, and a group of 8 lanes finishes a wide test sooner than one lane
-// (2128 bits, 2048 numbers: 11.7 against 40.2 ms) at 59% of its throughput.  While few sides scan, the group's one-wave residency (an
-// eighth of the one-lane count) still gives each side a deep window, so such
-// rounds run on the 8-lane variants when the one-lane variant is wide.
-constexpr uint32_t LATENCY_SIDES = 128;      // scanning sides at most
+// (2128 bits, 2048 numbers: 11.7 against 40.2 ms) at 59% of its throughput.
+// While at most LATENCY_SIDES sides scan, rounds run on the 8-lane variants
+// (when the one-lane variant is wide): many short rounds, each side a
+// shallower window.  1511#/2190 at half of capacity: p50/p99 per m 50/139 ms
+// against 126/206 with one-lane rounds; at 0.8 of capacity 144/316 against
+// 142/297 (above the threshold the one-lane rounds take over again).
+constexpr uint32_t LATENCY_SIDES = 1024;     // scanning sides at most
 constexpr int LATENCY_MIN_DIGITS = 52;       // one-lane variant at least this wide
 
 enum : uint32_t {
