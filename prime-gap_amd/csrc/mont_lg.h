@@ -1,5 +1,5 @@
 // mont_lg.h -- Montgomery arithmetic on numbers spread over a group of G
-// lanes (G = 4 or 8 lanes per number), radix 2^28.
+// lanes (G = 4, 8 or 16 lanes per number), radix 2^28.
 //
 // Used where one lane cannot hold a number (above ~2240 bits the three
 // L-digit arrays of the one-lane kernel no longer fit the register file).
@@ -21,7 +21,9 @@
 //
 // G = 8 halves the registers per lane (5S + temporaries), which lets a
 // 4247-bit number run at more than one wavefront per SIMD without the
-// accumulation registers spilling to AGPRs.
+// accumulation registers spilling to AGPRs.  G = 16 (a whole DPP row) halves
+// a test's latency again at a lower throughput; the streaming scan uses it
+// while few m's are in flight.
 //
 // Reference counterpart: CGBN's warp-cooperative cgbn_mont_sqr with
 // TPI threads per instance (miller_rabin.h:157-211, TPI = 8 in
@@ -38,9 +40,10 @@ constexpr uint32_t MASK = (1u << RB) - 1;
 
 __device__ __forceinline__ uint64_t mad(uint32_t a, uint32_t b, uint64_t c) { return (uint64_t)a * b + c; }
 
-// DPP control codes (GFX9 DPP16).
+// DPP control codes (GFX9 DPP16; row_newbcast from gfx90a on).
 constexpr int DPP_ROW_SHL1 = 0x101, DPP_ROW_SHL4 = 0x104, DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR4 = 0x114,
-              DPP_ROW_HALF_MIRROR = 0x141;
+              DPP_ROW_MIRROR = 0x140, DPP_ROW_HALF_MIRROR = 0x141, DPP_ROW_NEWBCAST0 = 0x150,
+              DPP_ROW_NEWBCAST15 = 0x15F;
 
 // Lane g of quad receives lane SRC's value.
 template <int P0, int P1, int P2, int P3>
@@ -52,9 +55,11 @@ __device__ __forceinline__ uint32_t qbcast(uint32_t v) { return qperm<SRC, SRC, 
 
 // Group lane 0's value to every lane of the group.  For G = 8 the quad
 // broadcast is followed by a 4-lane shift written only to the upper quad of
-// each group (banks 1 and 3 of the 16-lane row).
+// each group (banks 1 and 3 of the 16-lane row); a group of 16 is the whole
+// DPP row, one row_newbcast.
 template <int G>
 __device__ __forceinline__ uint32_t gbcast0(uint32_t v) {
+    if constexpr (G == 16) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, DPP_ROW_NEWBCAST0, 0xF, 0xF, false);
     const uint32_t q = qbcast<0>(v);
     if constexpr (G == 4) return q;
     return (uint32_t)__builtin_amdgcn_update_dpp((int)q, (int)q, DPP_ROW_SHR4, 0xF, 0xA, false);
@@ -62,6 +67,7 @@ __device__ __forceinline__ uint32_t gbcast0(uint32_t v) {
 // Group lane G-1's value to every lane of the group.
 template <int G>
 __device__ __forceinline__ uint32_t gbcast_top(uint32_t v) {
+    if constexpr (G == 16) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, DPP_ROW_NEWBCAST15, 0xF, 0xF, false);
     const uint32_t q = qbcast<3>(v);
     if constexpr (G == 4) return q;
     return (uint32_t)__builtin_amdgcn_update_dpp((int)q, (int)q, DPP_ROW_SHL4, 0xF, 0x5, false);
@@ -76,21 +82,25 @@ template <int G>
 __device__ __forceinline__ uint64_t gup64(uint64_t v) {
     return (uint64_t)gup<G>((uint32_t)v) | ((uint64_t)gup<G>((uint32_t)(v >> 32)) << 32);
 }
-// Reductions over the group: butterflies within the quad, then (G = 8) the
-// mirror of the 8-lane half-row, which pairs every lane with the other quad.
+// Reductions over the group: butterflies within the quad, then (G >= 8) the
+// mirror of the 8-lane half-row, which pairs every lane with the other quad,
+// then (G = 16) the mirror of the row, which pairs the two halves.
 template <int G>
 __device__ __forceinline__ uint32_t gmax(uint32_t v) {
     v = max(v, qperm<1, 0, 3, 2>(v));
     v = max(v, qperm<2, 3, 0, 1>(v));
-    if constexpr (G == 8)
+    if constexpr (G >= 8)
         v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_ROW_HALF_MIRROR, 0xF, 0xF, false));
+    if constexpr (G == 16)
+        v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_ROW_MIRROR, 0xF, 0xF, false));
     return v;
 }
 template <int G>
 __device__ __forceinline__ uint32_t gor(uint32_t v) {
     v |= qperm<1, 0, 3, 2>(v);
     v |= qperm<2, 3, 0, 1>(v);
-    if constexpr (G == 8) v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_ROW_HALF_MIRROR, 0xF, 0xF, false);
+    if constexpr (G >= 8) v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_ROW_HALF_MIRROR, 0xF, 0xF, false);
+    if constexpr (G == 16) v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_ROW_MIRROR, 0xF, 0xF, false);
     return v;
 }
 
@@ -272,7 +282,7 @@ __device__ __forceinline__ void canon(uint32_t (&x)[S], uint32_t g) {
 template <int S, int G, bool LAZY = false>
 __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t np, uint32_t g,
                                          uint32_t *stage, uint32_t s = 0) {
-    static_assert(S <= 32 && (G == 4 || G == 8), "accumulator bound; groups tile a 16-lane DPP row");
+    static_assert(S <= 32 && (G == 4 || G == 8 || G == 16), "accumulator bound; groups tile a 16-lane DPP row");
     constexpr int ST = stage_stride(S);
     uint64_t t[S];
 #pragma unroll

@@ -763,10 +763,13 @@ class Scanner {
         const uint32_t sbits = select_bits(KIND_SPRP, bits, conf);
         if ((st = e->pick(KIND_FERMAT2, bits, &cf))) return st;
         if (confirm_rounds && (st = e->pick(KIND_SPRP, sbits, &cc))) return st;
-        if (2 * na <= LATENCY_SIDES && !e->opt.lanes_per_number && !e->opt.radix && cf.v->lanes == 1 &&
-            cf.v->digits >= LATENCY_MIN_DIGITS) {
-            const Variant *lf = find_variant(KIND_FERMAT2, (int)bits, 28, 8);
-            const Variant *lc = confirm_rounds ? find_variant(KIND_SPRP, (int)sbits, 28, 8) : nullptr;
+        if (2 * na <= LATENCY_SIDES && !e->opt.lanes_per_number && !e->opt.radix) {
+            // wide one-lane numbers move to 8 lanes, 8-lane numbers to 16
+            const int lanes = cf.v->lanes == 1 && cf.v->digits >= LATENCY_MIN_DIGITS ? 8
+                              : cf.v->lanes == 8                                       ? 16
+                                                                                       : 0;
+            const Variant *lf = lanes ? find_variant(KIND_FERMAT2, (int)bits, 28, lanes) : nullptr;
+            const Variant *lc = lanes && confirm_rounds ? find_variant(KIND_SPRP, (int)sbits, 28, lanes) : nullptr;
             if (lf && (!confirm_rounds || (lc && lc->scan))) {
                 cf = Choice{lf, 0};
                 cc = Choice{lc, 0};

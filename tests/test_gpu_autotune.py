@@ -21,7 +21,7 @@ def test_autotune_table_and_parity(engine, tmp_path):
     seen = {(r["test"], r["bits"]) for r in table}
     assert seen == {(k, w) for k in (pgap.FERMAT2, pgap.SPRP) for w in WIDTHS}
     for r in table:
-        assert r["lanes"] in (1, 4, 8) and r["tests_per_s"] > 0 and r["cap"] >= 0
+        assert r["lanes"] in (1, 4, 8, 16) and r["tests_per_s"] > 0 and r["cap"] >= 0
         # the chosen variant covers the width
         limit = r["radix"] * r["digits"] - (4 if r["test"] == pgap.FERMAT2 else 8)   # tuned with base 3
         assert limit >= r["bits"], r

@@ -158,6 +158,28 @@ def test_octet_variants_random_sizes(engine8, bits):
         assert engine8.last_launch().lanes_per_number == 8
 
 
+@pytest.mark.parametrize("bits", [700, 2128, 3000, 4250, 4470])
+def test_row_group_variants(bits):
+    # sixteen lanes (a whole DPP row) per number, 10 digits per lane: up to
+    # 4476 bits Fermat, 4472 strong PRP with bases up to 8
+    rng = random.Random(bits + 16)
+    vals = [rng.getrandbits(bits) | (1 << (bits - 1)) | 1 for _ in range(64)]
+    vals += [(1 << 2281) - 1, (1 << 1279) - 1, (1 << 4096) + 1, 341, 561]
+    x = vals[0]
+    while not ol.probab_prime(x):
+        x += 2
+    vals.append(x)
+    with pgap.Engine(0, lanes=16) as e:
+        got = check(e, vals)
+        assert got[-1] == 1 and list(got[64:67]) == [1, 1, 1]
+        assert e.last_launch().lanes_per_number == 16
+        got = check(e, vals[:24] + vals[64:], pgap.SPRP, (3, 5))
+        assert e.last_launch().lanes_per_number == 16
+        c = candidates(3001, 2190, 1, 100_000, 40000, 32, 19)   # 4231-4248-bit family
+        check(e, c)
+        check(e, c[:8], pgap.SPRP, (2, 3))
+
+
 def test_octet_variants_special(engine8):
     vals = [(1 << 2281) - 1, (1 << 3217) - 1, (1 << 4253) - 1, (1 << 4096) + 1, (1 << 4253) + 2, (1 << 4363) - 1]
     got = check(engine8, vals)
