@@ -193,7 +193,7 @@ size_t pgap_scan_in_flight(const pgap_engine *e);
 pgap_status pgap_scan_stats(const pgap_engine *e, uint64_t *tests_issued, uint64_t *rounds, uint32_t *slots);
 
 /* Autotuner.  For each width in bits[0..n), times every kernel family that
- * covers it (one lane per number; a group of 4 or of 8 lanes per number;
+ * covers it (one lane per number; a group of 4, 8 or 16 lanes per number;
  * radix 2^28) at its smallest covering size and under a few occupancy caps,
  * for both tests, and keeps the fastest.  The table replaces the engine's
  * current one (later batches and scans use it) and, when path is non-NULL,

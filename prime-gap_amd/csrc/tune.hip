@@ -1,5 +1,5 @@
 // tune.hip -- the autotuner: for each number width, time every kernel family
-// (one lane per number, a group of 4 or of 8 lanes per number; radix 2^28) at its
+// (one lane per number, a group of 4, 8 or 16 lanes per number; radix 2^28) at its
 // smallest covering digit count and under a few occupancy caps, and keep the
 // fastest.  The table is plain text and is loaded by pgap_create through
 // pgap_options.tuning_path.

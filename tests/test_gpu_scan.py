@@ -78,6 +78,32 @@ def test_alone_equals_mixed(unknowns_907):
         assert a == b, m
 
 
+def test_alone_equals_mixed_across_round_variants():
+    # 1511#/2190 (2128 bits): 640 m's in flight are 1280 scanning sides, above
+    # the latency-first threshold, so the mix starts on one-lane rounds and
+    # ends on 8-lane ones; an m alone runs on 8-lane rounds throughout.  Rows
+    # must not depend on the variant a round used.
+    P, D, SL = 1511, 2190, 40000
+    ms = [m for m in range(1, 3000) if math.gcd(m, D) == 1][:640]
+    with pgap.Engine(0) as e:
+        fam = e.family(P, D, SL)
+        lists = e.sieve(fam, np.array(ms, dtype=np.uint64), 100_000_000, SL)
+        reqs = [(m, p, q) for m, (p, q) in zip(ms, lists)]
+        mixed = run_scan(e, fam, reqs)
+        alone = {}
+        for req in reqs[::97]:
+            alone.update(run_scan(e, fam, [req]))
+    assert len(mixed) == len(ms) and len(alone) == 7
+    for m, r in alone.items():
+        a = (r.prev_p, r.next_p, r.p_tests, r.n_tests, r.flags)
+        b = (mixed[m].prev_p, mixed[m].next_p, mixed[m].p_tests, mixed[m].n_tests, mixed[m].flags)
+        assert a == b, m
+    # spot-check the oracle on two of them
+    for m, p, q in reqs[:2]:
+        w = ol.test_interval(P, D, m, SL, p, q, kind=0)
+        assert (mixed[m].prev_p, mixed[m].next_p) == (w["prev_p"], w["next_p"]), m
+
+
 @pytest.mark.parametrize("confirm", [0, 1, 2])
 def test_confirmation_rejects_fermat_pseudoprimes(confirm):
     # K = 11#/1155 = 2 (init_K asserts K > 1), so N = 2m -+ x is small enough
