@@ -200,7 +200,7 @@ def test_fallback_beyond_sieve_length():
 
 
 def test_scan_2100_bit_family_small():
-    # 1511#/2190 (config-4 size) on a few m's sieved to 1e6
+    # 1511#/2190 (the 2128-bit serving family) on a few m's sieved to 1e6
     P, D, SL = 1511, 2190, 40000
     sv = ol.Sieve(P, D, SL, 1_000_000)
     ms = [m for m in range(1, 60) if math.gcd(m, D) == 1][:6]
