@@ -61,6 +61,7 @@ constexpr uint32_t MAX_SLOTS = 1u << 16;  // m's in flight (the HBM budget may a
 // against 126/206 with one-lane rounds; at 0.8 of capacity 144/316 against
 // 142/297 (above the threshold the one-lane rounds take over again).
 constexpr uint32_t LATENCY_SIDES = 1024;     // scanning sides at most
+constexpr uint32_t LATENCY_SIDES_ROW = 128;  // ... and at most this many for 16 lanes instead of 8
 constexpr int LATENCY_MIN_DIGITS = 52;       // one-lane variant at least this wide
 
 enum : uint32_t {
@@ -764,8 +765,10 @@ class Scanner {
         if ((st = e->pick(KIND_FERMAT2, bits, &cf))) return st;
         if (confirm_rounds && (st = e->pick(KIND_SPRP, sbits, &cc))) return st;
         if (2 * na <= LATENCY_SIDES && !e->opt.lanes_per_number && !e->opt.radix) {
-            // wide one-lane numbers move to 8 lanes, 8-lane numbers to 16
-            const int lanes = cf.v->lanes == 1 && cf.v->digits >= LATENCY_MIN_DIGITS ? 8
+            // wide one-lane numbers move to 8 lanes (16 while very few sides
+            // scan), 8-lane numbers to 16
+            const bool row = 2 * na <= LATENCY_SIDES_ROW;
+            const int lanes = cf.v->lanes == 1 && cf.v->digits >= LATENCY_MIN_DIGITS ? (row ? 16 : 8)
                               : cf.v->lanes == 8                                       ? 16
                                                                                        : 0;
             const Variant *lf = lanes ? find_variant(KIND_FERMAT2, (int)bits, 28, lanes) : nullptr;

@@ -180,6 +180,25 @@ def test_row_group_variants(bits):
         check(e, c[:8], pgap.SPRP, (2, 3))
 
 
+@pytest.mark.parametrize("bits", [1000, 2128, 2230])
+def test_row_group_variants_80_digits(bits):
+    # sixteen lanes, 5 digits per lane (80 digits): the scan's variant for
+    # 2128-bit numbers while very few m's are in flight
+    rng = random.Random(bits + 160)
+    vals = [rng.getrandbits(bits) | (1 << (bits - 1)) | 1 for _ in range(64)]
+    vals += [(1 << 2203) - 1, (1 << 1279) - 1, 341, 561]
+    x = vals[0]
+    while not ol.probab_prime(x):
+        x += 2
+    vals.append(x)
+    with pgap.Engine(0, lanes=16) as e:
+        got = check(e, vals)
+        assert got[-1] == 1 and list(got[64:66]) == [1, 1]
+        assert (e.last_launch().lanes_per_number, e.last_launch().digits) == (16, 80)
+        check(e, vals[:24] + vals[64:], pgap.SPRP, (3, 5))
+        assert (e.last_launch().lanes_per_number, e.last_launch().digits) == (16, 80)
+
+
 def test_octet_variants_special(engine8):
     vals = [(1 << 2281) - 1, (1 << 3217) - 1, (1 << 4253) - 1, (1 << 4096) + 1, (1 << 4253) + 2, (1 << 4363) - 1]
     got = check(engine8, vals)
