@@ -166,27 +166,39 @@ __device__ __forceinline__ uint32_t side_role(const SlotState &s, uint32_t side)
     return s.cursor[side] < s.len[side] ? 2 : 0;
 }
 
+// Share weight of a scanning side in a round: a side that has issued 128
+// candidates without a confirmed PRP (the tail of the gap distribution) gets
+// twice the share of the others, which cuts the rounds the longest gaps
+// take.  1511#/2190 at half of capacity: p99 134 -> 120 ms, p50 unchanged
+// (49 ms); 3001#/2190: p99 298 -> 261 ms, p50 133 -> 141 ms.
+__device__ __forceinline__ uint32_t side_weight(uint32_t cursor) { return 1u + min(cursor / 128u, 1u); }
+
 // ctr[1] = sides that run Fermat tests this round, ctr[4] = confirmation
-// rows (the Fermat depth divides the resident lanes the confirmations leave
-// among the scanning sides).
+// rows, ctr[5] = the scanning sides' total weight (the Fermat rows divide
+// the resident lanes the confirmations leave among the scanning sides).
 __global__ void count_kernel(const uint32_t *__restrict__ active, uint32_t na, const SlotState *__restrict__ st,
                              uint32_t nconf, uint32_t *__restrict__ ctr) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t role = t < 2 * na ? side_role(st[active[t >> 1]], t & 1u) : 0u;
+    const uint32_t side = t & 1u;
+    const SlotState *s = t < 2 * na ? &st[active[t >> 1]] : nullptr;
+    const uint32_t role = s ? side_role(*s, side) : 0u;
     const uint64_t bs = __ballot(role == 2), bc = __ballot(role == 1);
+    uint32_t w = role == 2 ? side_weight(s->cursor[side]) : 0u;
+    for (int off = 32; off > 0; off >>= 1) w += (uint32_t)__shfl_xor((int)w, off, 64);
     if ((threadIdx.x & 63u) == 0) {
         if (bs) atomicAdd(&ctr[1], (uint32_t)__popcll(bs));
         if (bc) atomicAdd(&ctr[4], (uint32_t)__popcll(bc) * nconf);
+        if (w) atomicAdd(&ctr[5], w);
     }
 }
 
 // One thread per (active slot, side): `nconf` confirmation items for a
 // pending Fermat hit, one per base (rows [0, cr), counter ctr[3]), or Fermat
-// items (rows cr + [0, ctr[0])).  Each scanning side takes an equal share of the
-// resident lanes the confirmations (padded to whole wavefronts of `cpad`
-// items) leave, at least one, capped by `depth` and by the `maxi` rows the
-// round holds: a grid larger than the resident lanes would run a second,
-// mostly empty wave and double the round's time.
+// items (rows cr + [0, ctr[0])).  Each scanning side takes its weighted share
+// (side_weight, total ctr[5]) of the resident lanes the confirmations (padded
+// to whole wavefronts of `cpad` items) leave, at least one, capped by `depth`
+// and by the `maxi` rows the round holds: a grid larger than the resident
+// lanes would run a second, mostly empty wave and double the round's time.
 __global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, SlotState *__restrict__ st,
                              uint32_t cr, uint32_t depth, uint32_t resident, uint32_t cpad, uint32_t maxi, Bases conf,
                              uint32_t *__restrict__ items, uint32_t *__restrict__ item_idx,
@@ -209,10 +221,14 @@ __global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, S
         return;
     }
     const uint32_t c = s.cursor[side], l = s.len[side];
-    const uint32_t ns = max(ctr[1], 1u);
+    const uint32_t ns = max(ctr[1], 1u), wsum = max(ctr[5], 1u), w = side_weight(c);
     const uint32_t used = (ctr[4] + cpad - 1) / cpad * cpad;
     const uint32_t room = resident > used ? resident - used : 0u;
-    const uint32_t d = min(min(depth, max(1u, room / ns)), max(1u, maxi / ns));
+    // weighted shares, at least one item each: the sum stays within room
+    // (when room >= ns) and within maxi (maxi >= 2 na >= ns)
+    const uint32_t share_room = room > ns ? (uint32_t)((uint64_t)(room - ns) * w / wsum) + 1u : 1u;
+    const uint32_t share_max = (uint32_t)((uint64_t)(maxi - ns) * w / wsum) + 1u;
+    const uint32_t d = min(min(depth, share_room), share_max);
     const uint32_t k = min(d, l - c);
     const uint32_t base = cr + atomicAdd(&ctr[0], k);
     for (uint32_t j = 0; j < k; ++j) {
