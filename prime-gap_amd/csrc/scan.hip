@@ -61,7 +61,7 @@ constexpr uint32_t MAX_SLOTS = 1u << 16;  // m's in flight (the HBM budget may a
 // against 126/206 with one-lane rounds; at 0.8 of capacity 144/316 against
 // 142/297 (above the threshold the one-lane rounds take over again).
 constexpr uint32_t LATENCY_SIDES = 1024;     // scanning sides at most
-constexpr uint32_t LATENCY_SIDES_ROW = 128;  // ... and at most this many for 16 lanes instead of 8
+constexpr uint32_t LATENCY_SIDES_ROW = 256;  // ... and at most this many for 16 lanes instead of 8
 constexpr int LATENCY_MIN_DIGITS = 52;       // one-lane variant at least this wide
 
 enum : uint32_t {
