@@ -305,7 +305,7 @@ def bench_stream(engine, pgap, n_m, load, seed, cpu_sample, threads, P4=1511, D4
     out = {
         "workload": f"P={P4} d={D4} SL={SL4}, m in seeded random order from [1,{m_hi}) coprime to d, unknowns "
                     f"sieved to {sieve_limit:.0e} ({np.mean([len(p) + len(q) for p, q in lists]):.0f} per m), Fermat-2 scan "
-                    f"with 2 strong-PRP confirmations, per-round depth sized to the resident lanes (cap 256)",
+                    f"with Baillie-PSW confirmation of each side's first PRP (strong base 2 + strong Lucas), per-round depth sized to the resident lanes (cap 256)",
         "m": n_m,
         "capacity_m_per_s": cap_mps,
         "useful_tests_per_s": useful / cap_s,

@@ -40,12 +40,21 @@ typedef enum {
     PGAP_E_STATE = -5       /* call not valid in the engine's current state */
 } pgap_status;
 
-/* The test a batch runs.  FERMAT2 is the reference kernel's ROUNDS=1 use
- * (gap_test_gpu.cu:81); SPRP runs a strong-probable-prime test for each base
- * in order (generate_primes order 2, 3, 5, ..., miller_rabin.h:269-286) and a
- * number passes when every base passes (results[i] = passed == rounds,
- * miller_rabin.h:381). */
-typedef enum { PGAP_FERMAT2 = 0, PGAP_SPRP = 1 } pgap_test;
+/* The test a batch runs.
+ *   FERMAT2  2^(N-1) == 1 (mod N): the scan's filter.
+ *   SPRP     a strong-probable-prime (Miller-Rabin) test for each base in
+ *            order; a number passes when every base passes.  The reference
+ *            GPU tester's test_runner_t(n, ROUNDS) runs this with the bases
+ *            generate_primes(ROUNDS) = 2, 3, 5, ... (miller_rabin.h:269-286,
+ *            317-319, results[i] = passed == rounds at :381); its build uses
+ *            ROUNDS = 1 (gap_test_gpu.cu:81), i.e. SPRP with bases {2}.
+ *   LUCAS    the strong Lucas test with Selfridge's method A parameters
+ *            (bases ignored).
+ *   BPSW     Baillie-PSW: SPRP base 2, then LUCAS (bases ignored); the
+ *            verdict of mpz_probab_prime_p(N, 25) != 0, the test of the
+ *            reference CPU tester (gap_test_common.cpp:144,154), for every N
+ *            that is not a BPSW pseudoprime (none is known). */
+typedef enum { PGAP_FERMAT2 = 0, PGAP_SPRP = 1, PGAP_LUCAS = 2, PGAP_BPSW = 3 } pgap_test;
 
 typedef struct {
     int device;               /* HIP device ordinal (the reference hard-codes 0, miller_rabin.h:326) */
@@ -84,7 +93,8 @@ pgap_status pgap_prp_batch_device(pgap_engine *e, uint32_t limbs, const uint32_t
 
 /* Mixed widths in one persistent launch.  Number i has limbs[i] words
  * starting at words + offset[i].  Numbers are grouped into size buckets on
- * the device and one work-queue kernel serves every bucket. */
+ * the device and one work-queue kernel serves every bucket.  FERMAT2 and
+ * SPRP only. */
 pgap_status pgap_prp_batch_mixed(pgap_engine *e, const uint32_t *words, const uint64_t *offset,
                                  const uint32_t *limbs, size_t count, pgap_test kind,
                                  const uint32_t *bases, uint32_t nbases, uint8_t *verdict);
@@ -195,7 +205,8 @@ pgap_status pgap_scan_stats(const pgap_engine *e, uint64_t *tests_issued, uint64
 /* Autotuner.  For each width in bits[0..n), times every kernel family that
  * covers it (one lane per number; a group of 4, 8 or 16 lanes per number;
  * radix 2^28) at its smallest covering size and under a few occupancy caps,
- * for both tests, and keeps the fastest.  The table replaces the engine's
+ * for each test (Fermat, strong PRP, strong Lucas), and keeps the fastest.
+ * The table replaces the engine's
  * current one (later batches and scans use it) and, when path is non-NULL,
  * is written there for pgap_options.tuning_path.  Text format, one line per
  * (test, width): "<test> <max_bits> <radix> <digits> <lanes> <cap> <tests/s>".
@@ -212,12 +223,23 @@ pgap_status pgap_autotune(pgap_engine *e, const uint32_t *bits, size_t n, const 
  * before the first submit. */
 pgap_status pgap_scan_side_skip(pgap_engine *e, int enable, float min_merit);
 
+/* What decides a side's prime.  Every scan tests candidates with FERMAT2;
+ * a side's first Fermat PRP is then confirmed, and one that fails the
+ * confirmation is treated as composite (the scan continues past it):
+ *   NONE   no confirmation: the first Fermat base-2 PRP;
+ *   SPRP2  a strong base-2 test: the reference GPU tester's verdict
+ *          (gap_test_gpu.cu:81 ROUNDS = 1, miller_rabin.h:213-245; the
+ *          textbook test, without that kernel's one extra squaring);
+ *   BPSW   strong base 2 plus the strong Lucas test: the verdict of
+ *          mpz_probab_prime_p(N, 25), gap_test_simple's row
+ *          (gap_test_common.cpp:144,154).  The default. */
+enum { PGAP_CONFIRM_NONE = 0, PGAP_CONFIRM_SPRP2 = 1, PGAP_CONFIRM_BPSW = 2 };
+
 /* Streaming knobs: the cap on speculative depth per side (a round gives
  * each unfinished side about resident-lanes / sides tests, never more than
- * this; default 256, at most 4096), and the Miller-Rabin confirmation rounds
- * on each side's first Fermat PRP (bases 3, 5, ...; default 2): a PRP failing
- * confirmation is treated as composite and the scan continues past it. */
-pgap_status pgap_scan_config(pgap_engine *e, uint32_t depth, uint32_t confirm_rounds);
+ * this; default 256, at most 4096), and the confirmation (PGAP_CONFIRM_*,
+ * default PGAP_CONFIRM_BPSW).  Must be called before the first submit. */
+pgap_status pgap_scan_config(pgap_engine *e, uint32_t depth, uint32_t confirm);
 
 #ifdef __cplusplus
 }

@@ -77,6 +77,107 @@ int or_sprp(const uint32_t *n, int limbs, uint32_t base) {
     return v;
 }
 
+/* Strong Lucas probable-prime test with Selfridge's method A parameters, the
+ * second half of the Baillie-PSW test that GMP 6.2's mpz_probab_prime_p runs
+ * (gap_test_common.cpp:144,154 call it with reps = 25: trial division,
+ * strong base 2, strong Lucas, then reps - 24 Miller-Rabin rounds).
+ *   D = first of 5, -7, 9, -11, 13, ... with Jacobi(D/N) = -1 (skipping a D
+ *       that N divides: N = |D| or a proper divisor of it; any other D with
+ *       Jacobi 0 shares a factor with N, so N is composite);
+ *   P = 1, Q = (1 - D)/4;  N + 1 = d * 2^s, d odd;
+ *   N passes if U_d == 0 or V_{d*2^r} == 0 (mod N) for some 0 <= r < s.
+ * 2 passes, other even N, N < 2 and perfect squares fail.  The D search is
+ * bounded at |D| < 2^25, beyond every non-square N's first suitable D under
+ * GRH (the least prime q with Jacobi(N/q) = -1 is below 2 ln(4N)^2); an N
+ * reaching the bound is reported composite.
+ * Computed with the textbook binary chain on (U_k, V_k, Q^k): doubling
+ * U_2k = U_k V_k, V_2k = V_k^2 - 2Q^k; increment U_k+1 = (P U_k + V_k)/2,
+ * V_k+1 = (D U_k + P V_k)/2 (halving mod N). */
+static int strong_lucas_z(const mpz_t N) {
+    if (mpz_cmp_ui(N, 2) < 0) return 0;
+    if (mpz_cmp_ui(N, 2) == 0) return 1;
+    if (mpz_even_p(N)) return 0;
+    if (mpz_perfect_square_p(N)) return 0;
+    long D = 0;
+    for (unsigned long a = 5; a < (1ul << 25); a += 2) {
+        const long Ds = (a & 2) ? -(long)a : (long)a;
+        const int j = mpz_si_kronecker(Ds, N);
+        if (j == -1) { D = Ds; break; }
+        if (j == 0 && !(mpz_cmp_ui(N, a) <= 0 && a % mpz_get_ui(N) == 0)) return 0;
+    }
+    if (D == 0) return 0;
+    const long Q = (1 - D) / 4;
+    mpz_t d, U, V, Qk, t, u;
+    mpz_inits(d, U, V, Qk, t, u, NULL);
+    mpz_add_ui(d, N, 1);
+    const mp_bitcnt_t s = mpz_scan1(d, 0);
+    mpz_tdiv_q_2exp(d, d, s);
+    /* k = 1: U_1 = 1, V_1 = P = 1, Q^1 = Q */
+    mpz_set_ui(U, 1);
+    mpz_set_ui(V, 1);
+    mpz_set_si(Qk, Q);
+    mpz_mod(Qk, Qk, N);
+    for (long i = (long)mpz_sizeinbase(d, 2) - 2; i >= 0; i--) {
+        mpz_mul(U, U, V);                 /* U_2k = U_k V_k */
+        mpz_mod(U, U, N);
+        mpz_mul(V, V, V);                 /* V_2k = V_k^2 - 2 Q^k */
+        mpz_submul_ui(V, Qk, 2);
+        mpz_mod(V, V, N);
+        mpz_mul(Qk, Qk, Qk);
+        mpz_mod(Qk, Qk, N);
+        if (mpz_tstbit(d, (mp_bitcnt_t)i)) {
+            mpz_add(t, U, V);             /* U_k+1 = (U_k + V_k)/2 (P = 1) */
+            mpz_mul_si(u, U, D);          /* V_k+1 = (D U_k + V_k)/2 */
+            mpz_add(u, u, V);
+            mpz_mod(t, t, N);
+            mpz_mod(u, u, N);
+            if (mpz_odd_p(t)) mpz_add(t, t, N);
+            if (mpz_odd_p(u)) mpz_add(u, u, N);
+            mpz_tdiv_q_2exp(U, t, 1);
+            mpz_tdiv_q_2exp(V, u, 1);
+            mpz_mul_si(Qk, Qk, Q);
+            mpz_mod(Qk, Qk, N);
+        }
+    }
+    int pass = mpz_sgn(U) == 0 || mpz_sgn(V) == 0;
+    for (mp_bitcnt_t r = 1; r < s && !pass; r++) {
+        mpz_mul(V, V, V);                 /* V_{2k} = V_k^2 - 2 Q^k */
+        mpz_submul_ui(V, Qk, 2);
+        mpz_mod(V, V, N);
+        mpz_mul(Qk, Qk, Qk);
+        mpz_mod(Qk, Qk, N);
+        pass = mpz_sgn(V) == 0;
+    }
+    mpz_clears(d, U, V, Qk, t, u, NULL);
+    return pass;
+}
+
+/* Baillie-PSW: strong base 2, then strong Lucas (2 itself passes).  For every
+ * N this equals mpz_probab_prime_p(N, reps) != 0 unless N is a BPSW
+ * pseudoprime (none is known; none below 2^64). */
+static int bpsw_z(const mpz_t N, mpz_t a, mpz_t b, mpz_t c) {
+    if (mpz_cmp_ui(N, 2) == 0) return 1;
+    return sprp_z(N, 2, a, b, c) && strong_lucas_z(N);
+}
+
+int or_strong_lucas(const uint32_t *n, int limbs) {
+    mpz_t N;
+    mpz_init(N);
+    import_limbs(N, n, limbs);
+    int v = strong_lucas_z(N);
+    mpz_clear(N);
+    return v;
+}
+
+int or_bpsw(const uint32_t *n, int limbs) {
+    mpz_t N, a, b, c;
+    mpz_inits(N, a, b, c, NULL);
+    import_limbs(N, n, limbs);
+    int v = bpsw_z(N, a, b, c);
+    mpz_clears(N, a, b, c, NULL);
+    return v;
+}
+
 /* gap_test_common.cpp:144,154: mpz_probab_prime_p(N, 25). */
 int or_probab_prime(const uint32_t *n, int limbs, int reps) {
     mpz_t N;
@@ -104,6 +205,10 @@ static void *verdict_worker(void *arg) {
         int v;
         if (j->kind == 0) {
             v = fermat2_z(N, a, b);
+        } else if (j->kind == 2) {
+            v = strong_lucas_z(N);
+        } else if (j->kind == 3) {
+            v = bpsw_z(N, a, b, c);
         } else {
             v = 1;
             for (int k = 0; k < j->nbases && v; k++) v = sprp_z(N, j->bases[k], a, b, c);
@@ -130,12 +235,6 @@ void or_verdicts(const uint32_t *n, size_t count, int limbs, int kind,
     for (int t = 0; t < threads; t++) pthread_join(tid[t], NULL);
     free(tid);
     free(jobs);
-}
-
-size_t or_bench_verdicts(const uint32_t *n, size_t count, int limbs, int kind, int threads) {
-    static const uint32_t mr_bases[2] = {3, 5};
-    or_verdicts(n, count, limbs, kind, mr_bases, 2, NULL, threads);
-    return count;
 }
 
 /* gap_common.cpp:116-121 */
@@ -192,16 +291,6 @@ static void add_si64(mpz_t z, int64_t x) {
     }
 }
 
-int or_mkx(uint32_t P, uint32_t D, uint64_t m, int64_t x, uint32_t *out, int cap) {
-    mpz_t c;
-    mpz_init(c);
-    center_of(c, P, D, m);
-    add_si64(c, x);
-    int r = (mpz_sgn(c) < 0) ? -1 : export_limbs(c, out, cap);
-    mpz_clear(c);
-    return r;
-}
-
 /* Largest prime strictly below z (z > 2); GMP 6.2.1 has no mpz_prevprime. */
 static void prevprime_z(mpz_t r, const mpz_t z) {
     mpz_sub_ui(r, z, 1);
@@ -236,24 +325,34 @@ int64_t or_next_prime_dist(uint32_t P, uint32_t D, uint64_t m, int64_t start) {
     return d;
 }
 
+/* The in-sieve test of or_test_interval by kind. */
+static int interval_test(const mpz_t t, int kind, mpz_t e, mpz_t r, mpz_t q) {
+    switch (kind) {
+    case 0: return mpz_probab_prime_p(t, 25) != 0;
+    case 1: return fermat2_z(t, e, r);
+    case 2: return sprp_z(t, 2, e, r, q);
+    default: return bpsw_z(t, e, r, q);
+    }
+}
+
 /* gap_test_common.cpp:128-181 */
 void or_test_interval(uint32_t P, uint32_t D, uint64_t m, uint32_t SL,
                       const int32_t *prev, int np, const int32_t *next, int nn,
                       int kind, int64_t *out) {
-    mpz_t center, t, e, r;
-    mpz_inits(center, t, e, r, NULL);
+    mpz_t center, t, e, r, q;
+    mpz_inits(center, t, e, r, q, NULL);
     center_of(center, P, D, m);
     int64_t prev_p = 0, next_p = 0, tests = 0, ptests = 0, ntests = 0;
     for (int i = 0; i < np; i++) {
         tests++; ptests++;
         mpz_sub_ui(t, center, (unsigned long)prev[i]);
-        int v = kind == 0 ? mpz_probab_prime_p(t, 25) : fermat2_z(t, e, r);
+        int v = interval_test(t, kind, e, r, q);
         if (v) { prev_p = prev[i]; break; }
     }
     for (int i = 0; i < nn; i++) {
         tests++; ntests++;
         mpz_add_ui(t, center, (unsigned long)next[i]);
-        int v = kind == 0 ? mpz_probab_prime_p(t, 25) : fermat2_z(t, e, r);
+        int v = interval_test(t, kind, e, r, q);
         if (v) { next_p = next[i]; break; }
     }
     int oos_prev = 0, oos_next = 0;
@@ -273,7 +372,7 @@ void or_test_interval(uint32_t P, uint32_t D, uint64_t m, uint32_t SL,
     }
     out[0] = prev_p; out[1] = next_p; out[2] = tests;
     out[3] = oos_prev; out[4] = oos_next; out[5] = ptests; out[6] = ntests;
-    mpz_clears(center, t, e, r, NULL);
+    mpz_clears(center, t, e, r, q, NULL);
 }
 
 /* ---- sieve: the unknown set combined_sieve writes ---- */
@@ -337,8 +436,6 @@ void or_sieve_free(or_sieve *s) {
     free(s->kmod);
     free(s);
 }
-
-size_t or_sieve_nprimes(const or_sieve *s) { return s ? s->nprimes : 0; }
 
 /* For the offsets x in [-SL, SL] that some prime in (lo, max_prime] removes:
  * x and the first such prime (in increasing order).  Returns the count, or

@@ -14,6 +14,13 @@
  *                            Fermat / strong pseudoprime lists, Mersenne and
  *                            Fermat numbers (tests/test_oracle.py).
  *   or_probab_prime        : brute force below 2^32, known primes/composites.
+ *   or_strong_lucas        : the strong Lucas pseudoprimes (Selfridge method A)
+ *                            below 10^5 of OEIS A217255 are exactly the
+ *                            composites it passes; every prime passes; an
+ *                            independent linear-recurrence evaluation of
+ *                            U_n, V_n in Python below 3000.
+ *   or_bpsw                : equals mpz_probab_prime_p != 0 on every N below
+ *                            2*10^5 and on the pseudoprime lists.
  *   or_prevprime/nextprime : brute force on small ranges; the record gap
  *                            11244911*809#/1841070 (-5788, +17354) from the
  *                            reference README.md:40-46.
@@ -40,9 +47,12 @@ extern "C" {
 int or_fermat2(const uint32_t *n, int limbs);                 /* 2^(N-1) mod N == 1 */
 int or_sprp(const uint32_t *n, int limbs, uint32_t base);     /* strong PRP, textbook */
 int or_probab_prime(const uint32_t *n, int limbs, int reps);  /* mpz_probab_prime_p */
+int or_strong_lucas(const uint32_t *n, int limbs);            /* strong Lucas, Selfridge A */
+int or_bpsw(const uint32_t *n, int limbs);                    /* strong base 2 + strong Lucas */
 
 /* Batched verdicts: n is row-major [count][limbs].  kind 0 = Fermat base 2,
- * kind 1 = strong PRP to each base in bases[0..nbases). threads <= 0 means 1. */
+ * kind 1 = strong PRP to each base in bases[0..nbases), kind 2 = strong
+ * Lucas, kind 3 = Baillie-PSW. threads <= 0 means 1. */
 void or_verdicts(const uint32_t *n, size_t count, int limbs, int kind,
                  const uint32_t *bases, int nbases, uint8_t *out, int threads);
 
@@ -51,9 +61,6 @@ void or_verdicts(const uint32_t *n, size_t count, int limbs, int kind,
 int or_K(uint32_t P, uint32_t D, uint32_t *out, int cap);
 /* ln(K) computed exactly as gap_common.cpp:100-104 (mpz_get_d_2exp). */
 double or_K_log(uint32_t P, uint32_t D);
-
-/* N = m*K + x as limbs (x may be negative); returns limb count or -1. */
-int or_mkx(uint32_t P, uint32_t D, uint64_t m, int64_t x, uint32_t *out, int cap);
 
 /* Distances from centre = m*K to the previous prime below centre - start and
  * the next prime above centre + start, restating the fallback of
@@ -67,8 +74,10 @@ int64_t or_next_prime_dist(uint32_t P, uint32_t D, uint64_t m, int64_t start);
  * unknown distances with mpz_probab_prime_p(N, 25), first PRP wins, fall back
  * beyond SL.  out[0]=prev_p out[1]=next_p out[2]=prp tests (including hits)
  * out[3]=prev fell back out[4]=next fell back out[5]=prev tests out[6]=next
- * tests.  kind 0 uses mpz_probab_prime_p(N,25) as the reference does; kind 1
- * uses Fermat base 2 (the engine's test) for the in-sieve scan. */
+ * tests.  The in-sieve test by kind: 0 mpz_probab_prime_p(N,25) as
+ * gap_test_simple does (gap_test_common.cpp:144,154); 1 Fermat base 2; 2 the
+ * strong base-2 test of the reference GPU tester (gap_test_gpu.cu:81 ROUNDS=1,
+ * miller_rabin.h:213-245 in its textbook form); 3 Baillie-PSW. */
 void or_test_interval(uint32_t P, uint32_t D, uint64_t m, uint32_t SL,
                       const int32_t *prev, int np, const int32_t *next, int nn,
                       int kind, int64_t *out);
@@ -81,14 +90,9 @@ or_sieve *or_sieve_new(uint32_t P, uint32_t D, uint32_t SL, uint64_t max_prime);
 or_sieve *or_sieve_new_range(uint32_t P, uint32_t D, uint32_t SL, uint64_t lo, uint64_t max_prime);
 void or_sieve_free(or_sieve *s);
 int or_sieve_m(const or_sieve *s, uint64_t m, int32_t *prev, int *np, int32_t *next, int *nn, int cap);
-size_t or_sieve_nprimes(const or_sieve *s);
 int or_sieve_removed(const or_sieve *s, uint64_t m, uint64_t lo, int64_t *xs, uint64_t *ps, int cap);
 
-/* CPU baselines, timed by the caller: a static partition over `threads`
- * pthreads.  Return the number of items done.  kind as in or_verdicts. */
-size_t or_bench_verdicts(const uint32_t *n, size_t count, int limbs, int kind, int threads);
-
-/* gap_test_simple over count m's (gap_test_simple.cpp:152-195 calling
+/* CPU baseline, timed by the caller: gap_test_simple over count m's (gap_test_simple.cpp:152-195 calling
  * test_interval_cpu), m's split statically over `threads` pthreads.  List i
  * is prev[prev_off[i] .. prev_off[i] + n_prev[i]) (likewise next).  kind as
  * in or_test_interval.  rows[3*i..] = prev_p, next_p, prp tests. */

@@ -246,6 +246,78 @@ pgap_status pgap_engine::launch_batch(const Choice &c, const uint32_t *d_n, uint
     return PGAP_OK;
 }
 
+// Rows whose number is 2: Baillie-PSW passes it, the strong base-2 test
+// (odd N only) does not.
+__global__ void two_kernel(const uint32_t *__restrict__ n, uint32_t limbs, uint32_t count,
+                           uint8_t *__restrict__ out) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= count) return;
+    const uint32_t *w = n + (size_t)t * limbs;
+    uint32_t hi = 0;
+    for (uint32_t j = 1; j < limbs; ++j) hi |= w[j];
+    if (w[0] == 2u && hi == 0) out[t] = 1;
+}
+
+// Baillie-PSW: the strong base-2 test on every row, the strong Lucas test on
+// the rows that passed (selected on the device), then 2 itself.
+pgap_status pgap_engine::launch_bpsw(const Choice &cs, const Choice &cl, const uint32_t *d_n, uint32_t limbs,
+                                     size_t count, uint8_t *d_out) {
+    pgap_status st;
+    if ((st = ensure(d_sel, std::min(count, kBatchChunk) * 4, "survivor index")) ||
+        (st = ensure(d_nsel, 64, "survivor count")))
+        return st;
+    hipError_t e = hipEventRecord(ev0, stream);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    Bases two{}, none{};
+    two.n = 1;
+    two.b[0] = 2;
+    uint32_t *sel = (uint32_t *)d_sel.p, *nsel = (uint32_t *)d_nsel.p;
+    uint32_t launches = 0;
+    for (size_t off = 0; off < count; off += kBatchChunk) {
+        const uint32_t n = (uint32_t)std::min(kBatchChunk, count - off);
+        const uint32_t *dn = d_n + off * limbs;
+        uint8_t *dout = d_out + off;
+        if ((e = cs.v->launch(dn, limbs, n, nullptr, nullptr, two, dout, cs.cap, stream)) != hipSuccess)
+            return hip_fail(e, "strong base-2 launch");
+        hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, nsel);
+        hipLaunchKernelGGL(select_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, dout, n, sel, nsel);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "survivor selection");
+        if ((e = cl.v->launch(dn, limbs, n, nsel, sel, none, dout, cl.cap, stream)) != hipSuccess)
+            return hip_fail(e, "strong Lucas launch");
+        hipLaunchKernelGGL(two_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, dn, limbs, n, dout);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "bpsw fix-up");
+        launches += 2;
+    }
+    e = hipEventRecord(ev1, stream);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    last.digits = (uint32_t)cl.v->digits;
+    last.radix_bits = (uint32_t)cl.v->radix;
+    last.lanes_per_number = (uint32_t)cl.v->lanes;
+    last.launches = launches;
+    last.tests = count;
+    last.graph = 0;
+    last.kernel_ms = -1.0;
+    timing_pending = true;
+    return PGAP_OK;
+}
+
+// Variant(s) for a batch of `kind` over numbers of up to `bits` bits, and the
+// launch: BPSW runs a strong base-2 variant and a strong Lucas variant.
+pgap_status pgap_engine::run_batch(pgap_test kind, const Bases &bs, uint32_t bits, const uint32_t *d_n,
+                                   uint32_t limbs, size_t count, uint8_t *d_out) {
+    bits = std::max(bits, 2u);
+    pgap_status st;
+    if (kind == PGAP_BPSW) {
+        Choice cs, cl;
+        if ((st = pick(KIND_SPRP, bits, &cs)) || (st = pick(KIND_LUCAS, bits, &cl))) return st;
+        return launch_bpsw(cs, cl, d_n, limbs, count, d_out);
+    }
+    const int k = kind == PGAP_LUCAS ? KIND_LUCAS : (int)kind;
+    Choice v;
+    if ((st = pick(k, select_bits(k, bits, bs), &v))) return st;
+    return launch_batch(v, d_n, limbs, count, bs, d_out);
+}
+
 void pgap_engine::collect_timing() {
     if (!timing_pending) return;
     float ms = 0.f;
@@ -257,7 +329,7 @@ void pgap_engine::collect_timing() {
 static pgap_status check_bases(pgap_engine *e, pgap_test kind, const uint32_t *bases, uint32_t nbases,
                                Bases *out) {
     memset(out, 0, sizeof *out);
-    if (kind == PGAP_FERMAT2) return PGAP_OK;
+    if (kind == PGAP_FERMAT2 || kind == PGAP_LUCAS || kind == PGAP_BPSW) return PGAP_OK;
     if (kind != PGAP_SPRP) {
         e->set_error("unknown test kind %d", (int)kind);
         return PGAP_E_ARG;
@@ -390,14 +462,12 @@ pgap_status pgap_prp_batch(pgap_engine *e, uint32_t limbs, const uint32_t *n, si
     if (st) return st;
     (void)hipSetDevice(e->opt.device);
     const uint32_t bits = host_max_bits(n, limbs, count);
-    Choice v;
-    if ((st = e->pick((int)kind, select_bits((int)kind, std::max(bits, 2u), bs), &v))) return st;
     const size_t in_bytes = count * limbs * sizeof(uint32_t);
     if ((st = e->ensure(e->d_n, in_bytes, "numbers")) || (st = e->ensure(e->d_out, count, "verdicts")))
         return st;
     hipError_t he = hipMemcpyAsync(e->d_n.p, n, in_bytes, hipMemcpyHostToDevice, e->stream);
     if (he != hipSuccess) return e->hip_fail(he, "hipMemcpyAsync H2D");
-    if ((st = e->launch_batch(v, (const uint32_t *)e->d_n.p, limbs, count, bs, (uint8_t *)e->d_out.p)))
+    if ((st = e->run_batch(kind, bs, bits, (const uint32_t *)e->d_n.p, limbs, count, (uint8_t *)e->d_out.p)))
         return st;
     he = hipMemcpyAsync(verdict, e->d_out.p, count, hipMemcpyDeviceToHost, e->stream);
     if (he != hipSuccess) return e->hip_fail(he, "hipMemcpyAsync D2H");
@@ -434,9 +504,7 @@ pgap_status pgap_prp_batch_device(pgap_engine *e, uint32_t limbs, const uint32_t
         if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
         if (he != hipSuccess) return e->hip_fail(he, "max bits");
     }
-    Choice v;
-    if ((st = e->pick((int)kind, select_bits((int)kind, std::max(max_bits, 2u), bs), &v))) return st;
-    return e->launch_batch(v, d_n, limbs, count, bs, d_verdict);
+    return e->run_batch(kind, bs, max_bits, d_n, limbs, count, d_verdict);
 }
 
 pgap_status pgap_sync(pgap_engine *e) {

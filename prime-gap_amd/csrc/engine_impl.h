@@ -125,6 +125,10 @@ struct pgap_engine {
     pgap_status pick(int kind, uint32_t bits, pgap::Choice *out);  // select(), error when none covers
     pgap_status launch_batch(const pgap::Choice &c, const uint32_t *d_n, uint32_t limbs, size_t count,
                              const pgap::Bases &bases, uint8_t *d_out);
+    pgap_status launch_bpsw(const pgap::Choice &cs, const pgap::Choice &cl, const uint32_t *d_n, uint32_t limbs,
+                            size_t count, uint8_t *d_out);
+    pgap_status run_batch(pgap_test kind, const pgap::Bases &bs, uint32_t bits, const uint32_t *d_n,
+                          uint32_t limbs, size_t count, uint8_t *d_out);
     void collect_timing();
     void scan_shutdown();
 };

@@ -8,17 +8,18 @@
 // log m) as float), StatsCounters (gap_test_common.cpp:29-124), argument
 // handling of Args::argparse for the testers (gap_common.cpp:805-1099).
 //
-// Differences by design: PRP tests run on the GPU (Fermat base 2 plus
-// strong-PRP confirmation of each side's first hit, bases 3 and 5 by
-// default) instead of mpz_probab_prime_p(N, 25); m's are scanned concurrently
-// and rows are printed in m order as they complete; malformed input is an
-// error message and exit status 1, not an assert.
+// Differences by design: PRP tests run on the GPU: Fermat base 2 on every
+// candidate, and each side's first hit confirmed by Baillie-PSW (strong base
+// 2 plus strong Lucas), the verdict mpz_probab_prime_p(N, 25) gives; m's are
+// scanned concurrently and rows are printed in m order as they complete;
+// malformed input is an error message and exit status 1, not an assert.
 //
 // --side-skip selects gap_test_gpu's order instead (gap_test_gpu.cu:591-617):
 // prev side first, and an m whose prev gap has merit below min_merit/2 - 2
 // ends without its next side (counted as "only prev_prime", no row).
 //
-// Extra flags: --depth N, --confirm N (scan knobs), --hbm-budget-mb N,
+// Extra flags: --depth N, --confirm bpsw|sprp2|none (scan knobs; sprp2 is
+// the reference GPU tester's strong base-2 verdict), --hbm-budget-mb N,
 // --results FILE (tab-separated P D m next_p prev_p merit, the columns of the
 // reference's result table, schema.sql:142-154), --dump-unknowns (parse and
 // print the lists, no GPU).
@@ -61,7 +62,7 @@ struct Args {
     float min_merit = 18;
     uint64_t mskip = 0;
     int verbose = 2;
-    uint32_t depth = 256, confirm = 2;
+    uint32_t depth = 256, confirm = PGAP_CONFIRM_BPSW;
     size_t hbm_budget = 0;
     std::string results, db;
     int top_percent = 0;
@@ -77,7 +78,8 @@ void usage(const char *name) {
     printf("[OPTIONALLY]\n  --min-merit <x>   only display gaps with merit > x (default 18)\n");
     printf("  --mskip <m>       skip m < mskip (partial resume)\n");
     printf("  --depth <n>       cap on speculative tests per side per round (default 256)\n");
-    printf("  --confirm <n>     strong-PRP bases confirming each side's first PRP (default 2)\n");
+    printf("  --confirm <test>  confirmation of each side's first Fermat PRP: bpsw (default, the\n"
+           "                    mpz_probab_prime_p verdict), sprp2 (strong base 2, gap_test_gpu's), none\n");
     printf("  --hbm-budget-mb <n>  device memory the scan may keep resident\n");
     printf("  --side-skip       gap_test_gpu order: prev side first, skip next if prev merit < min/2-2\n");
     printf("  --results <file>  append result rows (P D m next_p prev_p merit, tab separated)\n");
@@ -124,7 +126,15 @@ Args parse_args(int argc, char **argv) {
             case 16: a.mskip = strtoull(optarg, nullptr, 10); break;
             case 3: a.min_merit = (float)atof(optarg); break;
             case 20: a.depth = (uint32_t)atoi(optarg); break;
-            case 21: a.confirm = (uint32_t)atoi(optarg); break;
+            case 21:
+                if (!strcmp(optarg, "bpsw")) a.confirm = PGAP_CONFIRM_BPSW;
+                else if (!strcmp(optarg, "sprp2")) a.confirm = PGAP_CONFIRM_SPRP2;
+                else if (!strcmp(optarg, "none")) a.confirm = PGAP_CONFIRM_NONE;
+                else {
+                    printf("--confirm takes bpsw, sprp2 or none: %s\n", optarg);
+                    a.valid = false;
+                }
+                break;
             case 22: a.hbm_budget = (size_t)strtoull(optarg, nullptr, 10) << 20; break;
             case 23: a.results = optarg; break;
             case 24: a.dump = true; break;

@@ -17,7 +17,9 @@ struct Bases {
     uint32_t b[kMaxBases];
 };
 
-enum Kind { KIND_FERMAT2 = 0, KIND_SPRP = 1 };
+// FERMAT2 and SPRP have one-lane and lane-group variants; LUCAS (the strong
+// Lucas test, lucas_lg.h) has lane-group variants only.
+enum Kind { KIND_FERMAT2 = 0, KIND_SPRP = 1, KIND_LUCAS = 2 };
 
 // Batch launch: numbers are rows of `limbs` words; verdicts out.  A strong-PRP
 // launch tests one base, bases.b[0] (bases.n must be 1; callers run further
@@ -64,7 +66,7 @@ struct Variant {
     int lanes;       // lanes cooperating on one number
     int kind;        // Kind
     int max_bits;    // largest N the variant accepts: Fermat N < R/16; strong PRP N < R/2^8,
-                     // which covers bases up to 8 (see select_bits)
+                     // which covers bases up to 8 (see select_bits); strong Lucas N < R/4
     batch_launch_fn launch;
     int (*blocks_per_cu)();  // resident 256-thread workgroups per CU (occupancy query)
     scan_launch_fn scan = nullptr;  // set on strong-PRP variants (max_bits is the SPRP limit)

@@ -70,7 +70,7 @@ static pgap_status mixed_once(pgap_engine *e, const uint32_t *words, const uint6
             return PGAP_E_ARG;
         }
     } else if (kind != PGAP_FERMAT2) {
-        e->set_error("unknown test kind %d", (int)kind);
+        e->set_error("pgap_prp_batch_mixed runs FERMAT2 or SPRP (kind %d)", (int)kind);
         return PGAP_E_ARG;
     }
     // bucket per number: the smallest digit count whose bound covers it

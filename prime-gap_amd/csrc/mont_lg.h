@@ -314,5 +314,86 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S
         normalize<S, G>(t, x, g);
 }
 
+// x <- x * y * R^{-1} mod N, exact digits in and out (x, y < N in, < 2N out
+// for N < R/2: (xy + mN)/R < N^2/R + N).  The same row-wise CIOS as
+// sqr_redc with the rows x_i * y: x goes through the stage, y stays in
+// registers.  Accumulator bound as for sqr_redc with exact digits.
+template <int S, int G>
+__device__ __forceinline__ void mul_redc(uint32_t (&x)[S], const uint32_t (&y)[S], const uint32_t (&n)[S],
+                                         uint32_t np, uint32_t g, uint32_t *stage) {
+    static_assert(S <= 32 && (G == 4 || G == 8 || G == 16), "accumulator bound; groups tile a 16-lane DPP row");
+    constexpr int ST = stage_stride(S);
+    uint64_t t[S];
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+        t[p] = 0;
+        stage[g * ST + p] = x[p];
+    }
+    const bool top = (g == G - 1);
+    for (int o = 0; o < G; ++o) {
+        uint32_t xb[S];
+#pragma unroll
+        for (int u = 0; u < S; ++u) xb[u] = stage[o * ST + u];
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const uint32_t xi = xb[u];
+#pragma unroll
+            for (int j = 0; j < S; ++j) t[(u + j) % S] = mad(xi, y[j], t[(u + j) % S]);
+            const uint32_t m = gbcast0<G>(((uint32_t)t[u] * np) & MASK);
+#pragma unroll
+            for (int j = 0; j < S; ++j) t[(u + j) % S] = mad(m, n[j], t[(u + j) % S]);
+            t[(u + 1) % S] += t[u] >> RB;
+            const uint32_t lo = row_down((uint32_t)t[u] & MASK);
+            t[u] = top ? 0u : lo;
+        }
+    }
+    normalize<S, G>(t, x, g);
+}
+
+// a <- (a - c) mod N for a, c < N (exact digits).  a - c wraps to a - c + R
+// when it borrows; adding N then carries out of the top lane, which drops
+// the R.
+template <int S, int G>
+__device__ __forceinline__ void mod_sub(uint32_t (&a)[S], const uint32_t (&c)[S], const uint32_t (&n)[S],
+                                        uint32_t g) {
+    uint32_t u[S];
+    const uint32_t borrow = sub_n<S, G>(a, c, u, g);
+    uint64_t t[S];
+#pragma unroll
+    for (int p = 0; p < S; ++p) t[p] = (uint64_t)u[p] + (borrow ? n[p] : 0u);
+    normalize<S, G>(t, a, g);
+}
+
+template <int S, int G>
+__device__ __forceinline__ bool is_zero(const uint32_t (&a)[S]) {
+    uint32_t d = 0;
+#pragma unroll
+    for (int p = 0; p < S; ++p) d |= a[p];
+    return gor<G>(d) == 0;
+}
+
+// y <- y >> 1 over the group (the top lane shifts in zero).
+template <int S, int G>
+__device__ __forceinline__ void shr1(uint32_t (&y)[S], uint32_t g) {
+    uint32_t above = row_down(y[0] & 1u);  // lane g <- lane g+1's lowest bit
+    if (g == G - 1) above = 0;
+#pragma unroll
+    for (int p = 0; p < S - 1; ++p) y[p] = (y[p] >> 1) | ((y[p + 1] & 1u) << (RB - 1));
+    y[S - 1] = (y[S - 1] >> 1) | (above << (RB - 1));
+}
+
+// y <- y + 2^pos (exact digits out; the value stays below R).
+template <int S, int G>
+__device__ __forceinline__ void add_pow2(uint32_t (&y)[S], uint32_t pos, uint32_t g) {
+    const uint32_t d = pos / RB, v = 1u << (pos % RB);
+#pragma unroll
+    for (int p = 0; p < S; ++p)
+        if (g * S + (uint32_t)p == d) y[p] += v;
+    uint64_t t[S];
+#pragma unroll
+    for (int p = 0; p < S; ++p) t[p] = y[p];
+    normalize<S, G>(t, y, g);
+}
+
 }  // namespace lg
 }  // namespace pgap

@@ -3,6 +3,7 @@
 #pragma once
 #include <algorithm>
 #include "kernels.h"
+#include "lucas_lg.h"
 #include "prp_lg.h"
 
 namespace pgap {
@@ -40,10 +41,12 @@ __global__ void __launch_bounds__(256) prp_lg_kernel(const uint32_t *__restrict_
     const uint32_t row = d_idx ? d_idx[item] : item;
     const uint32_t *nw = n + (size_t)row * limbs;
     uint32_t v;
-    if (KIND == KIND_FERMAT2)
+    if constexpr (KIND == KIND_FERMAT2)
         v = lg::fermat2<S, G>(nw, limbs, g, stage);
-    else
+    else if constexpr (KIND == KIND_SPRP)
         v = lg::sprp<S, G>(nw, limbs, bases.b[0], g, stage);
+    else
+        v = lg::strong_lucas<S, G>(nw, limbs, g, stage);
     if (g == 0) out[row] = (uint8_t)v;
 }
 
@@ -128,6 +131,17 @@ struct RegisterLG {
         (register_variant(Variant{lg::RB, G * Ss, G, KIND_SPRP, lg::RB * G * Ss - 8, &launch_lg<Ss, G, KIND_SPRP>,
                                   &occupancy_lg<Ss, G, KIND_SPRP>, &launch_scan_lg<Ss, G>,
                                   &occupancy_scan_lg<Ss, G>}),
+         ...);
+    }
+};
+
+// Strong Lucas variants (batch launch only; the scan launches them for its
+// Baillie-PSW confirmation rows through the same entry point).
+template <int G, int... Ss>
+struct RegisterLucas {
+    RegisterLucas() {
+        (register_variant(Variant{lg::RB, G * Ss, G, KIND_LUCAS, lg::RB * G * Ss - 2, &launch_lg<Ss, G, KIND_LUCAS>,
+                                  &occupancy_lg<Ss, G, KIND_LUCAS>}),
          ...);
     }
 };

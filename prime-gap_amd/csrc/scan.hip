@@ -12,17 +12,20 @@
 // index of the first confirmed PRP) stay in HBM until the m finishes.  A
 // round tests, for every unfinished side, either the next `depth` unknowns
 // (Fermat base 2) or, when the side's first Fermat hit is awaiting
-// confirmation, that hit against the next strong-PRP base:
+// confirmation, that hit's confirmation tests (a strong base-2 test and, for
+// Baillie-PSW, a strong Lucas test, all in one round):
 //   issue -> materialise N = centre -+ x -> one grid running the round's
 //   confirmations and Fermat tests -> retire (first Fermat hit per side,
 //   confirmation stage) -> per-slot update, emitting finished m's and
 //   exhausted sides.
-// Confirmations ride in the next round's grid instead of a launch of their
-// own, so a round costs one test's latency.  A hit that fails a
-// confirmation (a base-2 pseudoprime) restarts its side just past it.
+// Strong-PRP confirmations ride in the next round's grid instead of a launch
+// of their own, so a round costs one test's latency; strong Lucas rows (lane
+// groups only) follow in a launch of their own on the same stream.  A hit
+// that fails a confirmation (a base-2 pseudoprime) restarts its side just
+// past it.
 // Every index below a side's first PRP has been tested in this or an earlier
-// round, so "first confirmed PRP" is exactly the reference's first
-// mpz_probab_prime_p hit; tests issued past it in the same round are
+// round, so with Baillie-PSW confirmation "first confirmed PRP" is the
+// reference's first mpz_probab_prime_p hit; tests issued past it in the same round are
 // speculative and are not counted (p_tests / n_tests count up to the hit).
 // A side whose unknowns are exhausted continues past SL with candidates that
 // have no factor below 2^14 -- the prevprime(centre - SL) / nextprime(centre
@@ -123,16 +126,17 @@ __global__ void scatter_kernel(const Upload *__restrict__ up, const uint32_t *__
     if (d.kind == 0) {
         uint32_t *c = center + (size_t)d.slot * cw;
         for (uint32_t j = threadIdx.x; j < cw; j += blockDim.x) c[j] = j < d.cwords ? data[d.off_center + j] : 0u;
-        for (uint32_t j = threadIdx.x; j < d.n0; j += blockDim.x) l0[j] = (int32_t)data[d.off_l0 + j];
-        for (uint32_t j = threadIdx.x; j < d.n1; j += blockDim.x) l1[j] = (int32_t)data[d.off_l1 + j];
+        // the host bounds n0, n1 by cap; the guard keeps a slot's lists in place regardless
+        for (uint32_t j = threadIdx.x; j < min(d.n0, cap); j += blockDim.x) l0[j] = (int32_t)data[d.off_l0 + j];
+        for (uint32_t j = threadIdx.x; j < min(d.n1, cap); j += blockDim.x) l1[j] = (int32_t)data[d.off_l1 + j];
         if (threadIdx.x == 0) {
             SlotState s;
             s.cursor[0] = s.cursor[1] = 0;
             s.found[0] = s.found[1] = NONE;
             s.cand[0] = s.cand[1] = NONE;
             s.stage[0] = s.stage[1] = 0;
-            s.len[0] = d.n0;
-            s.len[1] = d.n1;
+            s.len[0] = min(d.n0, cap);
+            s.len[1] = min(d.n1, cap);
             s.flags = d.init_flags;
             s.skip_below = d.skip_below;
             s.pad = 0;
@@ -143,10 +147,11 @@ __global__ void scatter_kernel(const Upload *__restrict__ up, const uint32_t *__
         int32_t *l = d.side ? l1 : l0;
         const uint32_t base = st[d.slot].len[d.side];  // read before the barrier below
         __syncthreads();
-        for (uint32_t j = threadIdx.x; j < d.n0; j += blockDim.x) l[base + j] = (int32_t)data[d.off_l0 + j];
+        const uint32_t n0 = base < cap ? min(d.n0, cap - base) : 0u;
+        for (uint32_t j = threadIdx.x; j < n0; j += blockDim.x) l[base + j] = (int32_t)data[d.off_l0 + j];
         __syncthreads();
         if (threadIdx.x == 0) {
-            st[d.slot].len[d.side] = base + d.n0;
+            st[d.slot].len[d.side] = base + n0;
             // both sides of one slot can be appended in the same launch (two
             // blocks): clear this side's bit without losing the other's
             atomicAnd(&st[d.slot].flags, ~(d.side ? F_NEXT_WAIT : F_PREV_WAIT));
@@ -192,17 +197,19 @@ __global__ void count_kernel(const uint32_t *__restrict__ active, uint32_t na, c
     }
 }
 
-// One thread per (active slot, side): `nconf` confirmation items for a
-// pending Fermat hit, one per base (rows [0, cr), counter ctr[3]), or Fermat
-// items (rows cr + [0, ctr[0])).  Each scanning side takes its weighted share
+// One thread per (active slot, side): for a pending Fermat hit, `conf.n`
+// strong-PRP confirmation items, one per base (rows [0, cr), counter ctr[3]),
+// plus a strong Lucas item when `lucas` is set (rows lr0 + [0, ctr[6])); or
+// Fermat items (rows cr + [0, ctr[0])).  Each scanning side takes its weighted share
 // (side_weight, total ctr[5]) of the resident lanes the confirmations (padded
 // to whole wavefronts of `cpad` items) leave, at least one, capped by `depth`
 // and by the `maxi` rows the round holds: a grid larger than the resident
 // lanes would run a second, mostly empty wave and double the round's time.
 __global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, SlotState *__restrict__ st,
                              uint32_t cr, uint32_t depth, uint32_t resident, uint32_t cpad, uint32_t maxi, Bases conf,
-                             uint32_t *__restrict__ items, uint32_t *__restrict__ item_idx,
-                             uint8_t *__restrict__ item_base, uint32_t *__restrict__ ctr) {
+                             uint32_t lr0, uint32_t lucas, uint32_t *__restrict__ items,
+                             uint32_t *__restrict__ item_idx, uint8_t *__restrict__ item_base,
+                             uint32_t *__restrict__ ctr) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= 2 * na) return;
     const uint32_t slot = active[t >> 1], side = t & 1u;
@@ -215,6 +222,11 @@ __global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, S
             items[r + j] = (slot << 1) | side;
             item_idx[r + j] = s.cand[side];
             item_base[r + j] = (uint8_t)conf.b[j];
+        }
+        if (lucas) {
+            const uint32_t l = lr0 + atomicAdd(&ctr[6], 1u);
+            items[l] = (slot << 1) | side;
+            item_idx[l] = s.cand[side];
         }
         s.stage[side] = 0;
         atomicOr(&s.flags, side ? F_NEXT_CONF : F_PREV_CONF);  // the other side's thread writes flags too
@@ -238,19 +250,20 @@ __global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, S
     s.cursor[side] = c + k;
 }
 
-// Rows in use this round: confirmations [0, ctr[3]), Fermat 2 na + [0, ctr[0]).
-__device__ __forceinline__ bool row_live(uint32_t t, uint32_t cr, const uint32_t *ctr) {
-    return t < cr ? t < ctr[3] : t - cr < ctr[0];
+// Rows in use this round: strong-PRP confirmations [0, ctr[3]), Fermat
+// cr + [0, ctr[0]), strong Lucas confirmations lr0 + [0, ctr[6]).
+__device__ __forceinline__ bool row_live(uint32_t t, uint32_t cr, uint32_t lr0, const uint32_t *ctr) {
+    return t < cr ? t < ctr[3] : (t < lr0 ? t - cr < ctr[0] : t - lr0 < ctr[6]);
 }
 
 // N = centre -+ x for every item, `w` words per row.
 __global__ void materialize_kernel(const uint32_t *__restrict__ items, const uint32_t *__restrict__ item_idx,
-                                   uint32_t cr, uint32_t nrows, const uint32_t *__restrict__ ctr,
+                                   uint32_t cr, uint32_t lr0, uint32_t nrows, const uint32_t *__restrict__ ctr,
                                    const uint32_t *__restrict__ center, uint32_t cw,
                                    const int32_t *__restrict__ lists, uint32_t cap, uint32_t *__restrict__ nbuf,
                                    uint32_t w) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= nrows || !row_live(t, cr, ctr)) return;
+    if (t >= nrows || !row_live(t, cr, lr0, ctr)) return;
     const uint32_t slot = items[t] >> 1, side = items[t] & 1u;
     const uint32_t x = (uint32_t)lists[((size_t)slot * 2 + side) * cap + item_idx[t]];
     const uint32_t *c = center + (size_t)slot * cw;
@@ -272,24 +285,23 @@ __global__ void materialize_kernel(const uint32_t *__restrict__ items, const uin
     }
 }
 
-// Confirmation rows count the bases their side's pending hit passed (the
+// Confirmation rows count the tests their side's pending hit passed (the
 // update decides); Fermat hits become the side's pending hit (or, with no
-// confirmation rounds, its PRP).  A side has either confirmation rows or
-// Fermat rows in a round, never both.
+// confirmation, its PRP).  A side has either confirmation rows or Fermat
+// rows in a round, never both.
 __global__ void retire_kernel(const uint32_t *__restrict__ items, const uint32_t *__restrict__ item_idx,
-                              uint32_t cr, uint32_t nrows, const uint8_t *__restrict__ verdict,
-                              const uint32_t *__restrict__ ctr, uint32_t confirm_rounds,
-                              SlotState *__restrict__ st) {
+                              uint32_t cr, uint32_t lr0, uint32_t nrows, const uint8_t *__restrict__ verdict,
+                              const uint32_t *__restrict__ ctr, uint32_t confirm, SlotState *__restrict__ st) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= nrows || !row_live(t, cr, ctr)) return;
+    if (t >= nrows || !row_live(t, cr, lr0, ctr)) return;
     const uint32_t slot = items[t] >> 1, side = items[t] & 1u;
     SlotState &s = st[slot];
-    if (t < cr) {
+    if (t < cr || t >= lr0) {
         if (verdict[t]) atomicAdd(&s.stage[side], 1u);
         return;
     }
     if (!verdict[t]) return;
-    atomicMin(confirm_rounds ? &s.cand[side] : &s.found[side], item_idx[t]);
+    atomicMin(confirm ? &s.cand[side] : &s.found[side], item_idx[t]);
 }
 
 __global__ void update_kernel(const uint32_t *__restrict__ active, uint32_t na, SlotState *__restrict__ st,
@@ -300,7 +312,7 @@ __global__ void update_kernel(const uint32_t *__restrict__ active, uint32_t na, 
     const uint32_t slot = active[t];
     SlotState &s = st[slot];
     uint32_t flags = s.flags, code = 0;
-    // a confirmation tested this round: every base passed -> the side's PRP;
+    // a confirmation tested this round: every test passed -> the side's PRP;
     // any failed (a base-2 pseudoprime) -> resume just past it
     for (uint32_t side = 0; side < 2; ++side) {
         const uint32_t conf = side ? F_NEXT_CONF : F_PREV_CONF;
@@ -373,7 +385,14 @@ struct SlotHost {
     double t_submit = 0;
     uint32_t bits = 0;             // bit length of the centre
     uint64_t room = 0;             // 2^bits - centre, saturated at 2^40
+    uint64_t small = 0;            // the centre when below 2^63, else 0
 };
+
+// The centre's value when it is below 2^63, else 0.
+static uint64_t small_value(const Words &c) {
+    if (bitlen(c) > 63) return 0;
+    return (uint64_t)(c.size() > 0 ? c[0] : 0) | ((uint64_t)(c.size() > 1 ? c[1] : 0) << 32);
+}
 
 // 2^bitlen(c) - c, saturated at 2^40: next-side candidates centre + x with
 // x < room keep the centre's bit length.
@@ -397,10 +416,11 @@ struct Staged {
 class Scanner {
   public:
     pgap_engine *e;
-    uint32_t depth = 256, confirm_rounds = 2;
+    uint32_t depth = 256, confirm = PGAP_CONFIRM_BPSW;
     bool side_skip = false;       // gap_test_gpu's prev-first order with side-skip
     float skip_min_merit = 0;     // min_merit / 2 - 2 (float, gap_test_gpu.cu:604)
     uint32_t S = 0, CW = 0, CAP = 0, W = 0, MAXI = 0;
+    uint32_t SL_MAX = 0;  // largest sieve length the slots' lists were sized for
     bool inited = false;
 
     // Shared with the client thread, under mu: staged requests, ready rows,
@@ -437,7 +457,7 @@ class Scanner {
     size_t up_cap = 0, updata_cap = 0;
 
     // pinned host staging
-    uint32_t *h_active = nullptr, *h_ctr = nullptr;
+    uint32_t *h_active = nullptr, *h_ctr = nullptr;  // h_ctr mirrors the 8 round counters
     Event *h_events = nullptr;
     Upload *h_up = nullptr;
     uint32_t *h_updata = nullptr;
@@ -546,7 +566,8 @@ class Scanner {
         }
         CW = kw + 2;   // m < 2^64
         W = CW + 1;    // room for the carry of + x
-        CAP = std::max<uint32_t>(sl, 1) + 4 * FALLBACK_CHUNK;
+        SL_MAX = std::max<uint32_t>(sl, 1);
+        CAP = SL_MAX + 4 * FALLBACK_CHUNK;
         // A round carries about as many Fermat tests as the chosen variant
         // keeps resident (the depth rule in round()), plus at most one per
         // side when depth 1 already exceeds that: the row buffer is sized by
@@ -560,10 +581,11 @@ class Scanner {
         }
         const size_t row_bytes = 4ull * W + 4 + 4 + 1;  // number, item, index, verdict
         // per slot: state, centre, lists, active entry, event, and per side
-        // one confirmation row (+ its base) and one Fermat row
-        const size_t nconf = std::max<uint32_t>(confirm_rounds, 1);
+        // one strong-PRP confirmation row (+ its base), one strong Lucas row
+        // and one Fermat row
+        const size_t nconf = 1, nlucas = confirm == PGAP_CONFIRM_BPSW ? 1 : 0;
         const size_t per_slot = sizeof(SlotState) + 4ull * CW + 8ull * CAP + 4 + sizeof(Event) +
-                                2ull * (nconf * (row_bytes + 1) + row_bytes);
+                                2ull * (nconf * (row_bytes + 1) + nlucas * row_bytes + row_bytes);
         size_t budget = e->opt.hbm_budget;
         if (budget == 0) {
             size_t fr = 0, tot = 0;
@@ -578,7 +600,8 @@ class Scanner {
             return PGAP_E_TOO_LARGE;
         }
         MAXI = (uint32_t)std::min<size_t>(extra + 2ull * S, 0x7fffffffu);
-        const size_t rows = 2ull * S * nconf + MAXI;  // confirmation rows, then Fermat rows
+        // strong-PRP confirmation rows, Fermat rows, strong Lucas rows
+        const size_t rows = 2ull * S * nconf + MAXI + 2ull * S * nlucas;
         pgap_status st;
         if ((st = dalloc(&d_state, S, "state")) || (st = dalloc(&d_center, (size_t)S * CW, "centres")) ||
             (st = dalloc(&d_lists, (size_t)S * 2 * CAP, "lists")) || (st = dalloc(&d_active, S, "active")) ||
@@ -586,7 +609,7 @@ class Scanner {
             (st = dalloc(&d_nbuf, rows * W, "numbers")) || (st = dalloc(&d_verdict, rows, "verdicts")) ||
             (st = dalloc(&d_base, 2ull * S * nconf, "confirmation bases")) || (st = dalloc(&d_ctr, 8, "counters")) ||
             (st = dalloc(&d_events, S, "events")) || (st = halloc(&h_active, S, "active")) ||
-            (st = halloc(&h_ctr, 4, "counters")) || (st = halloc(&h_events, S, "events"))) {
+            (st = halloc(&h_ctr, 8, "counters")) || (st = halloc(&h_events, S, "events"))) {
             release();
             return st;
         }
@@ -655,22 +678,31 @@ class Scanner {
         return PGAP_OK;
     }
 
-    // Screen offsets beyond SL for one exhausted side: no factor below 2^14.
-    std::vector<uint32_t> fallback_candidates(SlotHost &sh, int side) {
+    // Screen offsets beyond SL for one exhausted side: no factor below 2^14
+    // (N itself a prime below 2^14 passes).  The prev side stops at N = 3:
+    // the scan's tests take odd N, and a centre below SL + 4 is refused at
+    // submit, so a prime >= 3 always lies in that range.  `end_of_range` is
+    // set when the prev side has nothing left.
+    std::vector<uint32_t> fallback_candidates(SlotHost &sh, int side, bool *end_of_range) {
         const std::vector<uint32_t> &km = kmod_small(sh.fam);
         std::vector<uint32_t> out;
         uint32_t x = sh.fb_from[side];
-        const uint32_t end = x + FALLBACK_SPAN;
+        uint64_t end = (uint64_t)x + FALLBACK_SPAN;
+        if (side == 0 && sh.small) end = std::min<uint64_t>(end, sh.small - 2);  // x <= centre - 3
+        *end_of_range = side == 0 && sh.small && x >= end;
         std::vector<uint32_t> mr(fb_primes.size());
         for (size_t i = 0; i < fb_primes.size(); ++i)
             mr[i] = (uint32_t)((sh.m % fb_primes[i]) * km[i] % fb_primes[i]);
         for (; x < end && out.size() < FALLBACK_CHUNK; ++x) {
+            // N = m*K - x (prev) or m*K + x (next), exactly when the centre is small
+            const uint64_t N = sh.small ? (side ? sh.small + x : sh.small - x) : ~0ull;
             bool ok = true;
             for (size_t i = 0; i < fb_primes.size() && ok; ++i) {
                 const uint32_t p = fb_primes[i];
                 const uint32_t xr = x % p;
-                // N = m*K - x (prev) or m*K + x (next): divisible iff m*K == +-x mod p
-                ok = side ? ((mr[i] + xr) % p != 0) : (mr[i] != xr);
+                // divisible iff m*K == +-x mod p
+                const bool div = side ? ((mr[i] + xr) % p == 0) : (mr[i] == xr);
+                ok = !div || N == p;
             }
             if (ok) out.push_back(x);
         }
@@ -714,6 +746,7 @@ class Scanner {
             sh.t_submit = r.t_submit;
             sh.bits = bitlen(c);
             sh.room = room_above(c);
+            sh.small = small_value(c);
             Upload d{};
             d.slot = slot;
             d.kind = 0;
@@ -772,27 +805,33 @@ class Scanner {
             const uint64_t xmax = std::max<uint64_t>(sh.fam->SL, std::max(sh.fb_from[0], sh.fb_from[1]));
             bits = std::max(bits, sh.bits + (xmax >= sh.room ? 1u : 0u));
         }
+        // confirmation of a side's first Fermat PRP: a strong base-2 row in
+        // the round's grid, plus a strong Lucas row for Baillie-PSW
         Bases none{}, conf{};
-        static const uint32_t mr_bases[kMaxBases] = {3, 5, 7, 11, 13, 17, 19, 23};
-        conf.n = std::min<uint32_t>(confirm_rounds, kMaxBases);
-        for (uint32_t i = 0; i < conf.n; ++i) conf.b[i] = mr_bases[i];
-        Choice cf, cc;
-        const uint32_t sbits = select_bits(KIND_SPRP, bits, conf);
+        if (confirm != PGAP_CONFIRM_NONE) {
+            conf.n = 1;
+            conf.b[0] = 2;
+        }
+        const bool lucas = confirm == PGAP_CONFIRM_BPSW;
+        Choice cf, cc, cl;
         if ((st = e->pick(KIND_FERMAT2, bits, &cf))) return st;
-        if (confirm_rounds && (st = e->pick(KIND_SPRP, sbits, &cc))) return st;
+        if (conf.n && (st = e->pick(KIND_SPRP, bits, &cc))) return st;
+        if (lucas && (st = e->pick(KIND_LUCAS, bits, &cl))) return st;
         if (2 * na <= LATENCY_SIDES && !e->opt.lanes_per_number && !e->opt.radix) {
             // wide one-lane numbers move to 8 lanes (16 while very few sides
-            // scan), 8-lane numbers to 16
+            // scan), 8-lane numbers to 16; strong Lucas rows to 16 lanes
             const bool row = 2 * na <= LATENCY_SIDES_ROW;
             const int lanes = cf.v->lanes == 1 && cf.v->digits >= LATENCY_MIN_DIGITS ? (row ? 16 : 8)
                               : cf.v->lanes == 8                                       ? 16
                                                                                        : 0;
             const Variant *lf = lanes ? find_variant(KIND_FERMAT2, (int)bits, 28, lanes) : nullptr;
-            const Variant *lc = lanes && confirm_rounds ? find_variant(KIND_SPRP, (int)sbits, 28, lanes) : nullptr;
-            if (lf && (!confirm_rounds || (lc && lc->scan))) {
+            const Variant *lc = lanes && conf.n ? find_variant(KIND_SPRP, (int)bits, 28, lanes) : nullptr;
+            if (lf && (!conf.n || (lc && lc->scan))) {
                 cf = Choice{lf, 0};
                 cc = Choice{lc, 0};
             }
+            const Variant *ll = lucas ? find_variant(KIND_LUCAS, (int)bits, 28, 16) : nullptr;
+            if (ll) cl = Choice{ll, 0};
         }
         const Variant *vf = cf.v, *vc = cc.v;
         if (vc && !vc->scan) {
@@ -827,13 +866,16 @@ class Scanner {
         // Fermat rows: the per-side depth (set on the device from the number
         // of scanning sides) gives at most resident + one row per side
         const uint32_t maxi = (uint32_t)std::min<uint64_t>(MAXI, (uint64_t)resident + 2ull * na);
-        const uint32_t nrows = cr + maxi;
+        const uint32_t lr0 = cr + maxi;              // strong Lucas rows: one per side at most
+        const uint32_t nl = lucas ? 2 * na : 0;
+        const uint32_t nrows = lr0 + nl;
         const dim3 b256(256);
         hipLaunchKernelGGL(count_kernel, dim3((2 * na + 255) / 256), b256, 0, stream, d_active, na, d_state,
                            conf.n, d_ctr);
         hipLaunchKernelGGL(issue_kernel, dim3((2 * na + 255) / 256), b256, 0, stream, d_active, na, d_state, cr,
-                           depth_max(), resident, cpad, maxi, conf, d_items, d_idx, d_base, d_ctr);
-        hipLaunchKernelGGL(materialize_kernel, dim3((nrows + 255) / 256), b256, 0, stream, d_items, d_idx, cr,
+                           depth_max(), resident, cpad, maxi, conf, lr0, lucas ? 1u : 0u, d_items, d_idx, d_base,
+                           d_ctr);
+        hipLaunchKernelGGL(materialize_kernel, dim3((nrows + 255) / 256), b256, 0, stream, d_items, d_idx, cr, lr0,
                            nrows, d_ctr, d_center, CW, d_lists, CAP, d_nbuf, W);
         if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "scan issue");
         if (vc)
@@ -842,16 +884,22 @@ class Scanner {
             he = vf->launch(d_nbuf + (size_t)cr * W, W, maxi, d_ctr, nullptr, none, d_verdict + cr, cap_f,
                             stream);
         if (he != hipSuccess) return e->hip_fail(he, "scan test launch");
-        hipLaunchKernelGGL(retire_kernel, dim3((nrows + 255) / 256), b256, 0, stream, d_items, d_idx, cr, nrows,
-                           d_verdict, d_ctr, confirm_rounds, d_state);
+        if (lucas) {
+            // the round's strong Lucas rows (count on the device in ctr[6])
+            he = cl.v->launch(d_nbuf + (size_t)lr0 * W, W, nl, d_ctr + 6, nullptr, none, d_verdict + lr0, cl.cap,
+                              stream);
+            if (he != hipSuccess) return e->hip_fail(he, "scan strong Lucas launch");
+        }
+        hipLaunchKernelGGL(retire_kernel, dim3((nrows + 255) / 256), b256, 0, stream, d_items, d_idx, cr, lr0,
+                           nrows, d_verdict, d_ctr, confirm, d_state);
         hipLaunchKernelGGL(update_kernel, dim3((na + 255) / 256), b256, 0, stream, d_active, na, d_state, d_lists,
-                           CAP, conf.n, d_ctr, d_events);
+                           CAP, conf.n + (lucas ? 1u : 0u), d_ctr, d_events);
         if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "scan retire");
-        if ((he = hipMemcpyAsync(h_ctr, d_ctr, 16, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+        if ((he = hipMemcpyAsync(h_ctr, d_ctr, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream)) != hipSuccess ||
             (he = hipStreamSynchronize(stream)) != hipSuccess)
             return e->hip_fail(he, "scan round");
         const uint32_t nev = h_ctr[2];
-        const uint64_t issued = (uint64_t)h_ctr[0] + h_ctr[3];
+        const uint64_t issued = (uint64_t)h_ctr[0] + h_ctr[3] + h_ctr[6];
         if (nev) {
             if ((he = hipMemcpy(h_events, d_events, nev * sizeof(Event), hipMemcpyDeviceToHost)) != hipSuccess)
                 return e->hip_fail(he, "scan events");
@@ -866,8 +914,14 @@ class Scanner {
                 if (!(ev.code & (2u << side))) continue;
                 // exhausted: append screened candidates beyond SL (or past the last chunk)
                 sh.fb[side] = true;
-                std::vector<uint32_t> c = fallback_candidates(sh, side);
-                while (c.empty()) c = fallback_candidates(sh, side);
+                bool exhausted = false;
+                std::vector<uint32_t> c = fallback_candidates(sh, side, &exhausted);
+                while (c.empty() && !exhausted) c = fallback_candidates(sh, side, &exhausted);
+                if (c.empty()) {
+                    e->set_error("scan: m=%llu has no probable prime below its centre down to 3",
+                                 (unsigned long long)sh.m);
+                    return PGAP_E_STATE;
+                }
                 Upload d{};
                 d.slot = ev.slot;
                 d.kind = 1;
@@ -952,7 +1006,7 @@ pgap_status pgap_scan_side_skip(pgap_engine *e, int enable, float min_merit) {
     return PGAP_OK;
 }
 
-pgap_status pgap_scan_config(pgap_engine *e, uint32_t depth, uint32_t confirm_rounds) {
+pgap_status pgap_scan_config(pgap_engine *e, uint32_t depth, uint32_t confirm) {
     if (!e) return PGAP_E_ARG;
     if (!e->ready) return PGAP_E_STATE;
     Scanner *s = get_scanner(e);
@@ -960,12 +1014,12 @@ pgap_status pgap_scan_config(pgap_engine *e, uint32_t depth, uint32_t confirm_ro
         e->set_error("pgap_scan_config: must be called before the first submit");
         return PGAP_E_STATE;
     }
-    if (depth == 0 || depth > 4096 || confirm_rounds > (uint32_t)kMaxBases) {
-        e->set_error("pgap_scan_config: depth in [1,4096], confirm_rounds in [0,%d]", kMaxBases);
+    if (depth == 0 || depth > 4096 || confirm > PGAP_CONFIRM_BPSW) {
+        e->set_error("pgap_scan_config: depth in [1,4096], confirm one of PGAP_CONFIRM_NONE/SPRP2/BPSW");
         return PGAP_E_ARG;
     }
     s->depth = depth;
-    s->confirm_rounds = confirm_rounds;
+    s->confirm = confirm;
     return PGAP_OK;
 }
 
@@ -984,6 +1038,23 @@ pgap_status pgap_scan_submit(pgap_engine *e, const pgap_scan_req *req) {
     if ((size_t)f->K.size() + 2 > s->CW) {
         e->set_error("pgap_scan_submit: family registered after the scan started is wider than its slots");
         return PGAP_E_STATE;
+    }
+    if (f->SL > s->SL_MAX) {
+        e->set_error("pgap_scan_submit: family sieve length %u exceeds the %u the scan's slots were sized for "
+                     "(register every family before the first submit)",
+                     f->SL, s->SL_MAX);
+        return PGAP_E_STATE;
+    }
+    {
+        // the prev side's numbers m*K - x stay >= 4 (and its beyond-SL
+        // fallback finds a prime >= 3) only when m*K >= SL + 4
+        Words c = f->K;
+        mul_small(c, req->m);
+        const uint64_t cen = small_value(c);
+        if (cen && cen < (uint64_t)f->SL + 4) {
+            e->set_error("pgap_scan_submit: centre m*K = %llu below sieve length + 4", (unsigned long long)cen);
+            return PGAP_E_ARG;
+        }
     }
     if (req->n_prev > f->SL || req->n_next > f->SL) {
         e->set_error("pgap_scan_submit: more unknowns than the sieve length allows");

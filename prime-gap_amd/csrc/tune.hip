@@ -12,7 +12,8 @@
 //
 // Table format, one entry per line:
 //   <kind> <max_bits> <radix> <digits> <lanes> <blocks_cap> <tests_per_s>
-// kind 0 = Fermat base 2, 1 = strong PRP; blocks_cap 0 = natural occupancy.
+// kind 0 = Fermat base 2, 1 = strong PRP, 2 = strong Lucas; blocks_cap 0 =
+// natural occupancy.
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -71,7 +72,7 @@ extern "C" pgap_status pgap_autotune(pgap_engine *e, const uint32_t *bits, size_
             e->set_error("pgap_autotune: width %u outside [8, 65536]", b);
             return PGAP_E_ARG;
         }
-        for (int kind = 0; kind < 2; ++kind) {
+        for (int kind = KIND_FERMAT2; kind <= KIND_LUCAS; ++kind) {
             // smallest covering variant of each (radix, lanes) family
             std::vector<const Variant *> cand;
             for (size_t i = 0; i < variant_count(); ++i) {

@@ -26,6 +26,12 @@ PGAP_E_STATE = -5
 
 FERMAT2 = 0
 SPRP = 1
+LUCAS = 2   # strong Lucas, Selfridge's method A
+BPSW = 3    # strong base 2 + strong Lucas: mpz_probab_prime_p's verdict
+
+CONFIRM_NONE = 0   # a side's prime is its first Fermat base-2 PRP
+CONFIRM_SPRP2 = 1  # ... confirmed by a strong base-2 test (the reference GPU tester's verdict)
+CONFIRM_BPSW = 2   # ... confirmed by Baillie-PSW (gap_test_simple's verdict; the default)
 
 ROW_PREV_FALLBACK = 1
 ROW_NEXT_FALLBACK = 2
@@ -303,8 +309,9 @@ class Engine:
                                         npv.ctypes.data, nxt.ctypes.data, nnx.ctypes.data))
         return [(prev[i, : npv[i]].copy(), nxt[i, : nnx[i]].copy()) for i in range(n)]
 
-    def scan_config(self, depth: int, confirm_rounds: int):
-        self._check(self.lib.pgap_scan_config(self.h, depth, confirm_rounds))
+    def scan_config(self, depth: int, confirm: int = CONFIRM_BPSW):
+        """Speculative depth cap and the confirmation (CONFIRM_*)."""
+        self._check(self.lib.pgap_scan_config(self.h, depth, confirm))
 
     def scan_side_skip(self, enable: bool, min_merit: float = 18.0):
         """gap_test_gpu's prev-first order with side-skip below min_merit/2 - 2."""

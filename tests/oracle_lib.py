@@ -25,12 +25,13 @@ def load():
     lib.or_fermat2.argtypes = [vp, ctypes.c_int]
     lib.or_sprp.argtypes = [vp, ctypes.c_int, u32]
     lib.or_probab_prime.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.or_strong_lucas.argtypes = [vp, ctypes.c_int]
+    lib.or_bpsw.argtypes = [vp, ctypes.c_int]
     lib.or_verdicts.argtypes = [vp, sz, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int]
     lib.or_verdicts.restype = None
     lib.or_K.argtypes = [u32, u32, vp, ctypes.c_int]
     lib.or_K_log.argtypes = [u32, u32]
     lib.or_K_log.restype = ctypes.c_double
-    lib.or_mkx.argtypes = [u32, u32, u64, i64, vp, ctypes.c_int]
     lib.or_prev_prime_dist.argtypes = [u32, u32, u64, i64]
     lib.or_prev_prime_dist.restype = i64
     lib.or_next_prime_dist.argtypes = [u32, u32, u64, i64]
@@ -46,10 +47,6 @@ def load():
     lib.or_sieve_new_range.restype = vp
     lib.or_sieve_removed.argtypes = [vp, u64, u64, vp, vp, ctypes.c_int]
     lib.or_sieve_removed.restype = ctypes.c_int
-    lib.or_sieve_nprimes.argtypes = [vp]
-    lib.or_sieve_nprimes.restype = sz
-    lib.or_bench_verdicts.argtypes = [vp, sz, ctypes.c_int, ctypes.c_int, ctypes.c_int]
-    lib.or_bench_verdicts.restype = sz
     lib.or_bench_scan.argtypes = [u32, u32, u32, vp, sz, vp, vp, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
     lib.or_bench_scan.restype = sz
     _lib = lib
@@ -74,6 +71,16 @@ def sprp(x: int, base: int) -> int:
 def probab_prime(x: int, reps: int = 25) -> int:
     a = _limbs(x)
     return load().or_probab_prime(a.ctypes.data, len(a), reps)
+
+
+def strong_lucas(x: int) -> int:
+    a = _limbs(x)
+    return load().or_strong_lucas(a.ctypes.data, len(a))
+
+
+def bpsw(x: int) -> int:
+    a = _limbs(x)
+    return load().or_bpsw(a.ctypes.data, len(a))
 
 
 def verdicts(n: np.ndarray, kind: int = 0, bases=(), threads: int = 8) -> np.ndarray:

@@ -1,6 +1,7 @@
-"""Autotuner: the table it writes covers every requested width and both
-tests, and an engine created from that table gives the oracle's verdicts
-(the tuning changes speed only, never results)."""
+"""Autotuner: the table it writes covers every requested width and every
+test (Fermat, strong PRP, strong Lucas), and an engine created from that
+table gives the oracle's verdicts (the tuning changes speed only, never
+results)."""
 import random
 
 import numpy as np
@@ -19,11 +20,13 @@ def test_autotune_table_and_parity(engine, tmp_path):
     table = engine.autotune(WIDTHS, path)
     assert table == pgap.read_tuning(path)
     seen = {(r["test"], r["bits"]) for r in table}
-    assert seen == {(k, w) for k in (pgap.FERMAT2, pgap.SPRP) for w in WIDTHS}
+    assert seen == {(k, w) for k in (pgap.FERMAT2, pgap.SPRP, pgap.LUCAS) for w in WIDTHS}
     for r in table:
         assert r["lanes"] in (1, 4, 8, 16) and r["tests_per_s"] > 0 and r["cap"] >= 0
-        # the chosen variant covers the width
-        limit = r["radix"] * r["digits"] - (4 if r["test"] == pgap.FERMAT2 else 8)   # tuned with base 3
+        if r["test"] == pgap.LUCAS:
+            assert r["lanes"] in (8, 16)
+        # the chosen variant covers the width (strong PRP tuned with base 3)
+        limit = r["radix"] * r["digits"] - {pgap.FERMAT2: 4, pgap.SPRP: 8, pgap.LUCAS: 2}[r["test"]]
         assert limit >= r["bits"], r
 
     tuned = pgap.Engine(0, tuning_path=path)
@@ -36,7 +39,7 @@ def test_autotune_table_and_parity(engine, tmp_path):
                 x += 2
             vals.append(x)
             arr = pgap.to_limbs(vals, pgap.limbs_needed(max(vals)))
-            for kind, bases in ((pgap.FERMAT2, ()), (pgap.SPRP, (2, 3))):
+            for kind, bases in ((pgap.FERMAT2, ()), (pgap.SPRP, (2, 3)), (pgap.LUCAS, ())):
                 got = tuned.prp(arr, kind, bases)
                 want = ol.verdicts(arr, kind, bases)
                 assert np.array_equal(got, want), (bits, kind)

@@ -51,7 +51,13 @@ def test_scan_rows_907_match_oracle(unknowns_907):
     assert total == 2590  # README.md:241-262 prp-test total over the 53 m's
 
 
-@pytest.mark.parametrize("depth,confirm", [(1, 0), (3, 1), (16, 2)])
+# oracle in-sieve test per confirmation: Fermat base 2, strong base 2
+# (the reference GPU tester), mpz_probab_prime_p (gap_test_simple)
+ORACLE_KIND = {pgap.CONFIRM_NONE: 1, pgap.CONFIRM_SPRP2: 2, pgap.CONFIRM_BPSW: 0}
+
+
+@pytest.mark.parametrize("depth,confirm", [(1, pgap.CONFIRM_NONE), (3, pgap.CONFIRM_SPRP2),
+                                           (16, pgap.CONFIRM_BPSW)])
 def test_scan_depth_and_confirmation_do_not_change_rows(unknowns_907, depth, confirm):
     lines = lines_907(unknowns_907)[:20]
     with pgap.Engine(0) as e:
@@ -59,7 +65,7 @@ def test_scan_depth_and_confirmation_do_not_change_rows(unknowns_907, depth, con
         fam = e.family(907, 2190, 11000)
         rows = run_scan(e, fam, lines)
     for m, p, n in lines:
-        want = ol.test_interval(907, 2190, m, 11000, p, n, kind=0)
+        want = ol.test_interval(907, 2190, m, 11000, p, n, kind=ORACLE_KIND[confirm])
         assert (rows[m].prev_p, rows[m].next_p, rows[m].p_tests, rows[m].n_tests) == \
             (want["prev_p"], want["next_p"], want["p_tests"], want["n_tests"])
 
@@ -104,19 +110,19 @@ def test_alone_equals_mixed_across_round_variants():
         assert (mixed[m].prev_p, mixed[m].next_p) == (w["prev_p"], w["next_p"]), m
 
 
-@pytest.mark.parametrize("confirm", [0, 1, 2])
+@pytest.mark.parametrize("confirm", [pgap.CONFIRM_NONE, pgap.CONFIRM_SPRP2, pgap.CONFIRM_BPSW])
 def test_confirmation_rejects_fermat_pseudoprimes(confirm):
     # K = 11#/1155 = 2 (init_K asserts K > 1), so N = 2m -+ x is small enough
     # to place base-2 Fermat pseudoprimes in the lists: 404 - 63 = 341 =
     # 11*31 and 404 + 157 = 561 = 3*11*17 (a Carmichael number).  The
-    # Fermat-only scan (the reference's GPU tester, ROUNDS=1) stops on them;
-    # with confirmation the hit fails base 3 and the side resumes past it, as
-    # mpz_probab_prime_p would.
+    # Fermat-only scan stops on them; neither is a strong base-2 PRP, so the
+    # reference GPU tester's test (strong base 2, ROUNDS=1) and Baillie-PSW
+    # (mpz_probab_prime_p) both resume past them.
     P, D, SL, m = 11, 1155, 200, 202
     prev = [63, 65, 67, 71]    # 341 (psp), 339 = 3*113, 337 prime, 333
     nxt = [157, 159, 165]      # 561 (Carmichael), 563 prime, 569 prime
-    want = ol.test_interval(P, D, m, SL, prev, nxt, kind=0 if confirm else 1)
-    assert (want["prev_p"], want["next_p"]) == ((67, 159) if confirm else (63, 157))
+    want = ol.test_interval(P, D, m, SL, prev, nxt, kind=ORACLE_KIND[confirm])
+    assert (want["prev_p"], want["next_p"]) == ((63, 157) if confirm == pgap.CONFIRM_NONE else (67, 159))
     with pgap.Engine(0) as e:
         e.scan_config(4, confirm)
         fam = e.family(P, D, SL)
@@ -126,7 +132,58 @@ def test_confirmation_rejects_fermat_pseudoprimes(confirm):
         (want["prev_p"], want["next_p"], want["p_tests"], want["n_tests"], 0)
 
 
-@pytest.mark.parametrize("confirm", [0, 2])
+SPSP_235 = [25326001, 3215031751, 2152302898747, 3474749660383, 341550071728321]
+
+
+def test_bpsw_confirmation_rejects_strong_pseudoprimes():
+    # Strong pseudoprimes to bases 2, 3, 5 (OEIS A014233 terms) placed first
+    # on a side: K = 2, centre N - 1 (next side, x = 1) or N + 1 (prev side).
+    # The strong base-2 confirmation (the reference GPU tester's verdict)
+    # stops on them; Baillie-PSW rejects them as mpz_probab_prime_p does, and
+    # the side resumes to the next prime.
+    P, D, SL = 11, 1155, 200
+    odd = list(range(1, SL, 2))
+    reqs, psp_side = [], {}
+    for n in SPSP_235:
+        reqs.append(((n - 1) // 2, odd, odd))
+        psp_side[(n - 1) // 2] = "next_p"
+        reqs.append(((n + 1) // 2, odd, odd))
+        psp_side[(n + 1) // 2] = "prev_p"
+    rows = {}
+    for confirm in (pgap.CONFIRM_SPRP2, pgap.CONFIRM_BPSW):
+        with pgap.Engine(0) as e:
+            e.scan_config(64, confirm)
+            fam = e.family(P, D, SL)
+            rows[confirm] = run_scan(e, fam, reqs)
+    for m, p, q in reqs:
+        for confirm, kind in ((pgap.CONFIRM_SPRP2, 2), (pgap.CONFIRM_BPSW, 0)):
+            w = ol.test_interval(P, D, m, SL, p, q, kind=kind)
+            r = rows[confirm][m]
+            assert (r.prev_p, r.next_p, r.p_tests, r.n_tests) == \
+                (w["prev_p"], w["next_p"], w["p_tests"], w["n_tests"]), (m, confirm)
+        # the pseudoprime's side: distance 1 under the strong base-2 test only
+        side = psp_side[m]
+        assert getattr(rows[pgap.CONFIRM_SPRP2][m], side) == 1, m
+        assert getattr(rows[pgap.CONFIRM_BPSW][m], side) > 1, m
+
+
+def test_small_centres_fall_back_to_small_primes():
+    # K = 2, empty unknown lists: both sides fall back beyond SL.  Below
+    # 2^14 the fallback screen keeps N that are themselves small primes:
+    # m = 4 (centre 8, SL 4): prevprime(4) = 3, nextprime(12) = 13;
+    # m = 150 (centre 300, SL 100): 199 and 401.
+    P, D = 11, 1155
+    for m, SL, want in ((4, 4, (5, 5)), (150, 100, (101, 101))):
+        w = ol.test_interval(P, D, m, SL, [], [], kind=0)
+        assert (w["prev_p"], w["next_p"]) == want
+        with pgap.Engine(0) as e:
+            fam = e.family(P, D, SL)
+            r = run_scan(e, fam, [(m, [], [])])[m]
+        assert (r.prev_p, r.next_p) == want, m
+        assert r.flags == pgap.ROW_PREV_FALLBACK | pgap.ROW_NEXT_FALLBACK
+
+
+@pytest.mark.parametrize("confirm", [pgap.CONFIRM_NONE, pgap.CONFIRM_BPSW])
 def test_width_grows_past_a_power_of_two(confirm):
     # K = 2 and centres 2m just below 2^48: the next-side candidates 2m + x
     # reach 49 bits (the next primes of the first two m's are 2^48 + 21), one
@@ -209,7 +266,7 @@ def test_scan_2100_bit_family_small():
         fam = e.family(P, D, SL)
         rows = run_scan(e, fam, reqs)
     for m, p, n in reqs:
-        want = ol.test_interval(P, D, m, SL, p, n, kind=1)
+        want = ol.test_interval(P, D, m, SL, p, n, kind=0)
         r = rows[m]
         assert (r.prev_p, r.next_p, r.p_tests, r.n_tests) == \
             (want["prev_p"], want["next_p"], want["p_tests"], want["n_tests"]), m
@@ -226,7 +283,7 @@ def test_scan_4300_bit_family_small(lanes):
         fam = e.family(P, D, SL)
         rows = run_scan(e, fam, reqs)
     for m, p, n in reqs:
-        want = ol.test_interval(P, D, m, SL, p, n, kind=1)
+        want = ol.test_interval(P, D, m, SL, p, n, kind=0)
         r = rows[m]
         assert (r.prev_p, r.next_p, r.p_tests, r.n_tests) == \
             (want["prev_p"], want["next_p"], want["p_tests"], want["n_tests"]), m
@@ -264,9 +321,29 @@ def test_budget_and_admission(unknowns_907):
 def test_submit_validates_lists():
     with pgap.Engine(0) as e:
         fam = e.family(907, 2190, 11000)
+        tiny = e.family(11, 1155, 200)
+        # centre m*K = 200 below SL + 4: the prev side would reach N < 4
+        with pytest.raises(pgap.PgapError) as ei:
+            e.scan_submit(tiny, 100, np.array([1], np.int32), np.array([1], np.int32))
+        assert ei.value.status == pgap.PGAP_E_ARG
         with pytest.raises(pgap.PgapError):
             e.scan_submit(fam, 1, np.array([0], np.int32), np.array([], np.int32))
         with pytest.raises(pgap.PgapError):
             e.scan_submit(fam, 1, np.array([11001], np.int32), np.array([], np.int32))
         with pytest.raises(pgap.PgapError):
             e.scan_submit(fam + 5, 1, np.array([2], np.int32), np.array([], np.int32))
+
+
+def test_late_family_wider_than_the_slots_is_refused():
+    # The slots' unknown lists are sized at the first submit from the
+    # families registered by then; a later family with a longer sieve
+    # length is refused before anything is launched for it.
+    with pgap.Engine(0) as e:
+        fam = e.family(907, 2190, 5000)
+        assert e.scan_submit(fam, 1, np.array([4], np.int32), np.array([2], np.int32)) == pgap.PGAP_OK
+        big = e.family(907, 2190, 30000)
+        with pytest.raises(pgap.PgapError) as ei:
+            e.scan_submit(big, 1, np.array(list(range(2, 25000, 2)), np.int32), np.array([2], np.int32))
+        assert ei.value.status == pgap.PGAP_E_STATE
+        while e.scan_in_flight():
+            e.scan_poll(wait=True)

@@ -72,6 +72,100 @@ def test_probab_prime_small():
         assert (ol.probab_prime(n) > 0) == is_prime_td(n), n
 
 
+# OEIS A217255: strong Lucas pseudoprimes with Selfridge's method A, below 10^5
+STRONG_LUCAS_PSP = [5459, 5777, 10877, 16109, 18971, 22499, 24569, 25199, 40309, 58519, 75077, 97439]
+
+
+def jacobi_py(a: int, n: int) -> int:
+    """Jacobi symbol (a/n), n odd > 0 (textbook reciprocity loop)."""
+    a %= n
+    t = 1
+    while a:
+        while a % 2 == 0:
+            a //= 2
+            if n % 8 in (3, 5):
+                t = -t
+        a, n = n, a
+        if a % 4 == 3 and n % 4 == 3:
+            t = -t
+        a %= n
+    return t if n == 1 else 0
+
+
+def strong_lucas_linear(n: int) -> int:
+    """Strong Lucas test by direct linear recurrences U_k+1 = P U_k - Q U_k-1,
+    V likewise (no doubling formulas): independent of the oracle's chain."""
+    if n < 2 or (n % 2 == 0 and n != 2):
+        return 0
+    if n == 2:
+        return 1
+    if math.isqrt(n) ** 2 == n:
+        return 0
+    a = 5
+    while True:
+        D = -a if a % 4 == 3 else a
+        j = jacobi_py(D, n)
+        if j == -1:
+            break
+        if j == 0 and not (n <= a and a % n == 0):
+            return 0
+        a += 2
+    P, Q = 1, (1 - D) // 4
+    d, s = n + 1, 0
+    while d % 2 == 0:
+        d //= 2
+        s += 1
+    U = [0, 1]
+    V = [2, P % n]
+    for _ in range(2, d * (1 << (s - 1)) + 1):
+        U.append((P * U[-1] - Q * U[-2]) % n)
+        V.append((P * V[-1] - Q * V[-2]) % n)
+    if U[d] == 0 or V[d] == 0:
+        return 1
+    return 1 if any(V[d << r] == 0 for r in range(1, s)) else 0
+
+
+def test_strong_lucas_published_list_and_primes():
+    got = [n for n in range(1, 100000) if ol.strong_lucas(n) and not is_prime_td(n)]
+    assert got == STRONG_LUCAS_PSP
+    assert all(ol.strong_lucas(p) for p in primes_upto(100000))
+    # squares (no Selfridge D exists) and even numbers fail; so does 1
+    for n in (1, 4, 9, 25, 49, 1093 ** 2, 3511 ** 2, 10007 ** 2, (2 ** 61 - 1) ** 2):
+        assert ol.strong_lucas(n) == 0, n
+
+
+def test_strong_lucas_linear_recurrence():
+    for n in range(1, 2500):
+        assert ol.strong_lucas(n) == strong_lucas_linear(n), n
+    for n in STRONG_LUCAS_PSP[:4]:
+        assert strong_lucas_linear(n) == 1
+
+
+def test_bpsw_equals_probab_prime():
+    for n in range(0, 200000):
+        assert ol.bpsw(n) == (1 if ol.probab_prime(n) else 0), n
+    # strong base-2 pseudoprimes that also pass bases 3, 5, 7 (OEIS A014233
+    # terms) and the strong Lucas pseudoprimes: composite, rejected by BPSW
+    for n in [25326001, 3215031751, 2152302898747, 3474749660383, 341550071728321] + STRONG_LUCAS_PSP:
+        assert ol.bpsw(n) == 0 and ol.probab_prime(n) == 0, n
+    assert ol.sprp(341550071728321, 2) and ol.sprp(341550071728321, 13)
+    rng = random.Random(77)
+    for bits in (700, 1108, 2128, 4248):
+        vals = [rng.getrandbits(bits) | (1 << (bits - 1)) | 1 for _ in range(40)]
+        x = vals[0]
+        while not ol.probab_prime(x):
+            x += 2
+        vals.append(x)
+        for v in vals:
+            assert ol.bpsw(v) == (1 if ol.probab_prime(v) else 0)
+            assert ol.strong_lucas(v) == (1 if ol.probab_prime(v) else 0)
+    # composite Mersenne numbers are strong base-2 pseudoprimes; Lucas rejects them
+    for p in (11, 23, 1109, 2131):
+        assert ol.sprp(2 ** p - 1, 2) and ol.bpsw(2 ** p - 1) == 0
+    for p in (521, 607, 1279, 2203):
+        assert ol.bpsw(2 ** p - 1) == 1
+
+
 @pytest.mark.parametrize("bits", [33, 64, 65, 127, 700, 1108, 1500, 2130])
 def test_fermat2_vs_python_pow(bits):
     rng = random.Random(1000 + bits)
