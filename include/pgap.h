@@ -91,6 +91,21 @@ pgap_status pgap_prp_batch_device(pgap_engine *e, uint32_t limbs, const uint32_t
                                   pgap_test kind, const uint32_t *bases, uint32_t nbases,
                                   uint8_t *d_verdict, uint32_t max_bits);
 
+/* pgap_prp_batch_device for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  It calls no hipMalloc or
+ * hipFree: the survivor scratch that several SPRP bases and BPSW need is a
+ * fixed array of the library, one per GPU (4 MiB, held until the process
+ * ends), so count <= PGAP_NOALLOC_MAX_COUNT, and max_bits must be given
+ * (> 0).  Every call on one GPU shares that array: calls that overlap on a
+ * GPU (several engines or streams at once) race and must be serialised.
+ * Asynchronous on the engine's stream; pgap_sync() waits.  No timing is
+ * recorded for pgap_last_launch. */
+#define PGAP_NOALLOC_MAX_COUNT (1u << 20)
+pgap_status pgap_prp_batch_device_noalloc(pgap_engine *e, uint32_t limbs, const uint32_t *d_n, size_t count,
+                                          pgap_test kind, const uint32_t *bases, uint32_t nbases,
+                                          uint8_t *d_verdict, uint32_t max_bits);
+
 /* Mixed widths in one persistent launch.  Number i has limbs[i] words
  * starting at words + offset[i].  Numbers are grouped into size buckets on
  * the device and one work-queue kernel serves every bucket.  FERMAT2 and

@@ -64,6 +64,38 @@ __global__ void max_bits_kernel(const uint32_t *__restrict__ n, uint32_t limbs, 
     if ((threadIdx.x & 63) == 0 && b) atomicMax(out, b);
 }
 
+// ---- scratch of the allocation-free batch entry ---------------------------
+//
+// pgap_prp_batch_device_noalloc keeps its survivor index list and counter
+// (several strong-PRP bases, Baillie-PSW) in this array instead of engine
+// allocations.  Every GPU the code object loads on holds its own copy until
+// the process ends; pgap_create looks up that copy's address once per device
+// (hipGetSymbolAddress, with the device current) and every later use goes
+// through that address.  The host-side name `scratch` is not a device address
+// and is never passed to a kernel.
+constexpr size_t kNoAllocWords = (size_t)PGAP_NOALLOC_MAX_COUNT + 64;  // index list, then the counter
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+static std::mutex scratch_mu;
+static std::vector<uint32_t *> scratch_dev;  // per device ordinal
+
+// Called with `device` current.
+static hipError_t scratch_setup(int device) {
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if ((size_t)device >= scratch_dev.size()) scratch_dev.resize((size_t)device + 1, nullptr);
+    if (scratch_dev[(size_t)device]) return hipSuccess;
+    void *p = nullptr;
+    const hipError_t he = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (he != hipSuccess) return he;
+    scratch_dev[(size_t)device] = (uint32_t *)p;
+    return hipSuccess;
+}
+
+static uint32_t *scratch_of(int device) {
+    std::lock_guard<std::mutex> g(scratch_mu);
+    return (size_t)device < scratch_dev.size() ? scratch_dev[(size_t)device] : nullptr;
+}
+
 }  // namespace pgap
 
 using namespace pgap;
@@ -146,14 +178,16 @@ __global__ void zero_kernel(uint32_t *__restrict__ p) {
 // the rows that passed so far (selected on the device), so lanes that
 // already failed do not ride along through the later bases.  Every launch
 // tests one base (the kernels' contract).
+// `sel` (a row index per number of a chunk) and `nsel` (one counter) are the
+// survivor scratch, used when several bases are asked for.
 hipError_t pgap_engine::enqueue_batch(const Choice &c, const uint32_t *d_n, uint32_t limbs, size_t count,
-                                      const Bases &bases, uint8_t *d_out, uint32_t *launches) {
+                                      const Bases &bases, uint8_t *d_out, uint32_t *sel, uint32_t *nsel,
+                                      uint32_t *launches) {
     const Variant *v = c.v;
     const size_t chunk = kBatchChunk;  // grid dimensions are 32-bit: split very large batches
     const uint32_t nb = v->kind == KIND_SPRP ? bases.n : 1;
     Bases first = bases;
     if (v->kind == KIND_SPRP) first.n = 1;
-    uint32_t *sel = (uint32_t *)d_sel.p, *nsel = (uint32_t *)d_nsel.p;
     hipError_t e;
     for (size_t off = 0; off < count; off += chunk) {
         const uint32_t n = (uint32_t)std::min(chunk, count - off);
@@ -209,7 +243,8 @@ pgap_status pgap_engine::launch_batch(const Choice &c, const uint32_t *d_n, uint
             hipGraph_t g = nullptr;
             if ((e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
                 return hip_fail(e, "graph capture");
-            const hipError_t le = enqueue_batch(c, d_n, limbs, count, bases, d_out, &launches);
+            const hipError_t le = enqueue_batch(c, d_n, limbs, count, bases, d_out, (uint32_t *)d_sel.p,
+                                                (uint32_t *)d_nsel.p, &launches);
             e = hipStreamEndCapture(stream, &g);
             if (le != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
@@ -229,7 +264,8 @@ pgap_status pgap_engine::launch_batch(const Choice &c, const uint32_t *d_n, uint
         replayed = true;
         graph_launches = launches;
     } else {
-        if ((e = enqueue_batch(c, d_n, limbs, count, bases, d_out, &launches)) != hipSuccess)
+        if ((e = enqueue_batch(c, d_n, limbs, count, bases, d_out, (uint32_t *)d_sel.p, (uint32_t *)d_nsel.p,
+                               &launches)) != hipSuccess)
             return hip_fail(e, "kernel launch");
         graph = key;
     }
@@ -259,7 +295,31 @@ __global__ void two_kernel(const uint32_t *__restrict__ n, uint32_t limbs, uint3
 }
 
 // Baillie-PSW: the strong base-2 test on every row, the strong Lucas test on
-// the rows that passed (selected on the device), then 2 itself.
+// the rows that passed (selected on the device into `sel` / `nsel`), then 2
+// itself.
+hipError_t pgap_engine::enqueue_bpsw(const Choice &cs, const Choice &cl, const uint32_t *d_n, uint32_t limbs,
+                                     size_t count, uint8_t *d_out, uint32_t *sel, uint32_t *nsel,
+                                     uint32_t *launches) {
+    Bases two{}, none{};
+    two.n = 1;
+    two.b[0] = 2;
+    hipError_t e;
+    for (size_t off = 0; off < count; off += kBatchChunk) {
+        const uint32_t n = (uint32_t)std::min(kBatchChunk, count - off);
+        const uint32_t *dn = d_n + off * limbs;
+        uint8_t *dout = d_out + off;
+        if ((e = cs.v->launch(dn, limbs, n, nullptr, nullptr, two, dout, cs.cap, stream)) != hipSuccess) return e;
+        hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, nsel);
+        hipLaunchKernelGGL(select_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, dout, n, sel, nsel);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = cl.v->launch(dn, limbs, n, nsel, sel, none, dout, cl.cap, stream)) != hipSuccess) return e;
+        hipLaunchKernelGGL(two_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, dn, limbs, n, dout);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        *launches += 2;
+    }
+    return hipSuccess;
+}
+
 pgap_status pgap_engine::launch_bpsw(const Choice &cs, const Choice &cl, const uint32_t *d_n, uint32_t limbs,
                                      size_t count, uint8_t *d_out) {
     pgap_status st;
@@ -268,26 +328,10 @@ pgap_status pgap_engine::launch_bpsw(const Choice &cs, const Choice &cl, const u
         return st;
     hipError_t e = hipEventRecord(ev0, stream);
     if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
-    Bases two{}, none{};
-    two.n = 1;
-    two.b[0] = 2;
-    uint32_t *sel = (uint32_t *)d_sel.p, *nsel = (uint32_t *)d_nsel.p;
     uint32_t launches = 0;
-    for (size_t off = 0; off < count; off += kBatchChunk) {
-        const uint32_t n = (uint32_t)std::min(kBatchChunk, count - off);
-        const uint32_t *dn = d_n + off * limbs;
-        uint8_t *dout = d_out + off;
-        if ((e = cs.v->launch(dn, limbs, n, nullptr, nullptr, two, dout, cs.cap, stream)) != hipSuccess)
-            return hip_fail(e, "strong base-2 launch");
-        hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, nsel);
-        hipLaunchKernelGGL(select_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, dout, n, sel, nsel);
-        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "survivor selection");
-        if ((e = cl.v->launch(dn, limbs, n, nsel, sel, none, dout, cl.cap, stream)) != hipSuccess)
-            return hip_fail(e, "strong Lucas launch");
-        hipLaunchKernelGGL(two_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, dn, limbs, n, dout);
-        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "bpsw fix-up");
-        launches += 2;
-    }
+    if ((e = enqueue_bpsw(cs, cl, d_n, limbs, count, d_out, (uint32_t *)d_sel.p, (uint32_t *)d_nsel.p,
+                          &launches)) != hipSuccess)
+        return hip_fail(e, "Baillie-PSW launch");
     e = hipEventRecord(ev1, stream);
     if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
     last.digits = (uint32_t)cl.v->digits;
@@ -400,6 +444,11 @@ pgap_status pgap_create(const pgap_options *opt, pgap_engine **out) {
         *out = e;
         return PGAP_E_DEVICE;
     }
+    if ((he = scratch_setup(e->opt.device)) != hipSuccess) {
+        e->hip_fail(he, "scratch symbol address");
+        *out = e;
+        return PGAP_E_DEVICE;
+    }
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->opt.device) == hipSuccess && cus > 0)
         e->num_cus = cus;
@@ -505,6 +554,51 @@ pgap_status pgap_prp_batch_device(pgap_engine *e, uint32_t limbs, const uint32_t
         if (he != hipSuccess) return e->hip_fail(he, "max bits");
     }
     return e->run_batch(kind, bs, max_bits, d_n, limbs, count, d_verdict);
+}
+
+// The device batch without engine allocations: the same variants and
+// launches as pgap_prp_batch_device, with the survivor scratch in the
+// library's per-GPU array (no graph capture, no timing events).
+pgap_status pgap_prp_batch_device_noalloc(pgap_engine *e, uint32_t limbs, const uint32_t *d_n, size_t count,
+                                          pgap_test kind, const uint32_t *bases, uint32_t nbases,
+                                          uint8_t *d_verdict, uint32_t max_bits) {
+    if (!e) return PGAP_E_ARG;
+    if (!e->ready) return PGAP_E_STATE;
+    if (count == 0) return PGAP_OK;
+    if (!d_n || !d_verdict || limbs == 0 || max_bits == 0) {
+        e->set_error("pgap_prp_batch_device_noalloc: null buffer, zero limbs or max_bits 0 (the bound is required)");
+        return PGAP_E_ARG;
+    }
+    if (count > (size_t)PGAP_NOALLOC_MAX_COUNT) {
+        e->set_error("pgap_prp_batch_device_noalloc: %zu numbers exceed the %u its scratch holds (split the batch)",
+                     count, (unsigned)PGAP_NOALLOC_MAX_COUNT);
+        return PGAP_E_TOO_LARGE;
+    }
+    Bases bs;
+    pgap_status st = check_bases(e, kind, bases, nbases, &bs);
+    if (st) return st;
+    (void)hipSetDevice(e->opt.device);
+    uint32_t *scr = scratch_of(e->opt.device);
+    if (!scr) {
+        e->set_error("pgap_prp_batch_device_noalloc: no scratch address for device %d", e->opt.device);
+        return PGAP_E_STATE;
+    }
+    uint32_t *sel = scr, *nsel = scr + PGAP_NOALLOC_MAX_COUNT;
+    const uint32_t bits = std::max(max_bits, 2u);
+    uint32_t launches = 0;
+    hipError_t he;
+    if (kind == PGAP_BPSW) {
+        Choice cs, cl;
+        if ((st = e->pick(KIND_SPRP, bits, &cs)) || (st = e->pick(KIND_LUCAS, bits, &cl))) return st;
+        he = e->enqueue_bpsw(cs, cl, d_n, limbs, count, d_verdict, sel, nsel, &launches);
+    } else {
+        const int k = kind == PGAP_LUCAS ? KIND_LUCAS : (int)kind;
+        Choice v;
+        if ((st = e->pick(k, select_bits(k, bits, bs), &v))) return st;
+        he = e->enqueue_batch(v, d_n, limbs, count, bs, d_verdict, sel, nsel, &launches);
+    }
+    if (he != hipSuccess) return e->hip_fail(he, "pgap_prp_batch_device_noalloc launch");
+    return PGAP_OK;
 }
 
 pgap_status pgap_sync(pgap_engine *e) {

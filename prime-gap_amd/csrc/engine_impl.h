@@ -103,7 +103,10 @@ struct pgap_engine {
     pgap::BatchGraph graph;       // steady-state replay of a repeated batch launch
     uint32_t graph_launches = 0;  // kernel launches inside the captured graph
     hipError_t enqueue_batch(const pgap::Choice &c, const uint32_t *d_n, uint32_t limbs, size_t count,
-                             const pgap::Bases &bases, uint8_t *d_out, uint32_t *launches);
+                             const pgap::Bases &bases, uint8_t *d_out, uint32_t *sel, uint32_t *nsel,
+                             uint32_t *launches);
+    hipError_t enqueue_bpsw(const pgap::Choice &cs, const pgap::Choice &cl, const uint32_t *d_n, uint32_t limbs,
+                            size_t count, uint8_t *d_out, uint32_t *sel, uint32_t *nsel, uint32_t *launches);
     std::vector<pgap::Family> families;  // appended only by pgap_family (client thread)
     pgap::Scanner *scanner = nullptr;  // owned; released by scan_shutdown()
 

@@ -57,7 +57,9 @@ EXPORTED = (
     "pgap_last_launch", "pgap_family", "pgap_scan_submit", "pgap_scan_poll",
     "pgap_scan_in_flight", "pgap_scan_config", "pgap_build_candidates_device", "pgap_filter_candidates",
     "pgap_sieve", "pgap_scan_stats", "pgap_autotune", "pgap_scan_side_skip",
+    "pgap_prp_batch_device_noalloc",
 )
+NOALLOC_MAX_COUNT = 1 << 20
 
 
 class PgapError(RuntimeError):
@@ -137,6 +139,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.pgap_version.restype = ctypes.c_char_p
     lib.pgap_prp_batch.argtypes = [vp, u32, vp, sz, ctypes.c_int, vp, u32, vp]
     lib.pgap_prp_batch_device.argtypes = [vp, u32, vp, sz, ctypes.c_int, vp, u32, vp, u32]
+    lib.pgap_prp_batch_device_noalloc.argtypes = [vp, u32, vp, sz, ctypes.c_int, vp, u32, vp, u32]
     lib.pgap_prp_batch_mixed.argtypes = [vp, vp, vp, vp, sz, ctypes.c_int, vp, u32, vp]
     lib.pgap_sync.argtypes = [vp]
     lib.pgap_last_launch.argtypes = [vp, ctypes.POINTER(LaunchInfo)]
@@ -242,6 +245,13 @@ class Engine:
         b = np.ascontiguousarray(bases, dtype=np.uint32)
         self._check(self.lib.pgap_prp_batch_device(self.h, limbs, d_n, count, kind,
                                                    b.ctypes.data if len(b) else None, len(b), d_out, max_bits))
+
+    def prp_device_noalloc(self, d_n: int, limbs: int, count: int, d_out: int, kind: int = FERMAT2,
+                           bases: Sequence[int] = (), max_bits: int = 0) -> int:
+        """pgap_prp_batch_device_noalloc; returns the status (PGAP_OK or an error)."""
+        b = np.ascontiguousarray(bases, dtype=np.uint32)
+        return self.lib.pgap_prp_batch_device_noalloc(self.h, limbs, d_n, count, kind,
+                                                      b.ctypes.data if len(b) else None, len(b), d_out, max_bits)
 
     def prp_mixed(self, values: Sequence[np.ndarray], kind: int = FERMAT2, bases: Sequence[int] = ()) -> np.ndarray:
         """values: list of 1-D uint32 limb arrays of varying length."""
