@@ -7,10 +7,14 @@ every N free of prime factors <= 1e6 (screened on the device), 2^20
 candidates (N ~ 1108 bits), seeded.  One step = one Fermat base-2 test of
 every candidate, inputs resident in HBM.  Reported: tests/s over exactly
 --steps steps, the dominant kernel's roofline against the int32
-multiply-add peak, and the GMP oracle timed on the host on a bounded sample
-of the same candidates (cpu_baseline).
+multiply-add rate measured in the same run (pgap_mad_peak) and against the
+nominal one, and the GMP oracle timed on the host on a bounded sample of the
+same candidates (cpu_baseline).  `extra` holds the other widths, the
+mixed-width batch and the serving scans (503#/210 plain file, 1511#/2190,
+3001#/2190).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --mad-peak      (only the multiply-add rate)
 """
 from __future__ import annotations
 
@@ -33,6 +37,37 @@ X_RANGE = 15000
 SIEVE_LIMIT = 1_000_000
 # One v_mad_u64_u32 per lane per clock: 256 CUs x 64 lanes x 2.4 GHz.
 PEAK_INT32_MAC = 256 * 64 * 2.4e9
+MAD_PEAK_ITERS = 16384
+
+
+def measured_peak(engine, reps=3):
+    """Sustained v_mad_u64_u32 rate of this device (best of `reps` launches)."""
+    runs = [engine.mad_peak(MAD_PEAK_ITERS) for _ in range(reps)]
+    best = max(runs, key=lambda r: r["mac_per_s"])
+    return {"mac_per_s": best["mac_per_s"], "kernel_ms": best["kernel_ms"], "macs_per_launch": best["macs"],
+            "runs_mac_per_s": [r["mac_per_s"] for r in runs]}
+
+
+def host_cpus():
+    """Threads the CPU baseline uses: the CPUs this process may run on, capped
+    by OMP_NUM_THREADS (the host's share per GPU); plus what the host is."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    cap = os.environ.get("OMP_NUM_THREADS")
+    threads = min(allowed, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else allowed
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, {"threads_used": threads, "host_nproc": os.cpu_count(), "affinity_cpus": allowed,
+                     "omp_num_threads": cap, "cpu_model": model}
 
 
 def draw_candidates(engine, fam, count, seed):
@@ -75,7 +110,7 @@ def primorial_over(P, D):
     return math.prod(p for p in range(2, P + 1) if all(p % q for q in range(2, int(p ** 0.5) + 1))) // D
 
 
-def bench_batch_2180(engine, hipmem, pgap, count, steps, seed):
+def bench_batch_2180(engine, hipmem, pgap, count, steps, seed, peak_mac):
     """The 2180-bit width of the metric: N = m*1511#/2190 + x, m ~ U[1, 1e6)
     coprime to 2190, x ~ U[-20000, 20000], no prime factor <= 1e6."""
     P2, D2 = 1511, 2190
@@ -114,7 +149,9 @@ def bench_batch_2180(engine, hipmem, pgap, count, steps, seed):
         "tests_per_s": count / avg,
         "kernel_ms_avg": avg * 1e3,
         "kernel": f"radix-2^{info.radix_bits} x {info.digits} digits",
-        "roofline_frac": 2 * L32 * L32 * e_mean * count / avg / PEAK_INT32_MAC,
+        "roofline_frac": 2 * L32 * L32 * e_mean * count / avg / peak_mac,
+        "roofline_frac_of_nominal": 2 * L32 * L32 * e_mean * count / avg / PEAK_INT32_MAC,
+        "roofline_frac_container_72_limbs": 2 * 72 * 72 * e_mean * count / avg / peak_mac,
         "work_per_test": f"2*L^2*e, L={L32}, e={e_mean:.1f}",
     }
 
@@ -216,7 +253,7 @@ def bench_mixed(engine, hipmem, pgap, count, seed):
     }
 
 
-def bench_batch_4300(engine, hipmem, pgap, count, seed):
+def bench_batch_4300(engine, hipmem, pgap, count, seed, peak_mac):
     """Large operands: N = m*3001#/2190 + x, m ~ U[1, 1e5), x ~ U[-40000,
     40000], no factor <= 1e6 (4231-4248 bits, quad-per-number variants)."""
     rng = np.random.default_rng(seed)
@@ -238,25 +275,37 @@ def bench_batch_4300(engine, hipmem, pgap, count, seed):
         "tests_per_s": count / avg,
         "kernel_ms_avg": avg * 1e3,
         "kernel": f"radix-2^{info.radix_bits} x {info.digits} digits, {info.lanes_per_number} lanes/number",
-        "roofline_frac": 2 * L32 * L32 * e_mean * count / avg / PEAK_INT32_MAC,
+        "roofline_frac": 2 * L32 * L32 * e_mean * count / avg / peak_mac,
+        "roofline_frac_of_nominal": 2 * L32 * L32 * e_mean * count / avg / PEAK_INT32_MAC,
+        "roofline_frac_container_136_limbs": 2 * 136 * 136 * e_mean * count / avg / peak_mac,
+        "work_per_test": f"2*L^2*e, L={L32}, e={e_mean:.1f}",
     }
 
 
 def bench_stream(engine, pgap, n_m, load, seed, cpu_sample, threads, P4=1511, D4=2190, SL4=40000, m_hi=200000,
-                 sieve_limit=1_000_000_000):
+                 sieve_limit=1_000_000_000, all_m=False, cpu_info=None):
     """Serving: P=1511, d=2190, SL=40000, m's in seeded random order from
     [1, 200000) coprime to d, each with its unknowns sieved to 1e9 (device
     sieve, untimed).  Phase 1 saturates the engine (capacity); phase 2 feeds
     Poisson arrivals at `load` x capacity and records arrival-to-row latency.
-    The large-operand serving case passes P=3001, m < 1e5."""
+    The large-operand serving case passes P=3001, m < 1e5.  all_m: every m
+    coprime to d below m_hi, in both phases (the plain 503#/210 file, whose
+    m's are few), with every row checked against the CPU baseline's."""
     fam = engine.family(P4, D4, SL4)
     rng = np.random.default_rng(seed)
     pool = np.array([m for m in range(1, m_hi) if math.gcd(m, D4) == 1], dtype=np.uint64)
-    order = rng.permutation(pool)[: 2 * n_m]
+    if all_m:
+        n_m = len(pool)
+        order = rng.permutation(pool)
+    else:
+        order = rng.permutation(pool)[: 2 * n_m]
     t0 = time.perf_counter()
     lists = engine.sieve(fam, order, sieve_limit, SL4)
     sieve_s = time.perf_counter() - t0
-    cap_set, lat_set = list(range(n_m)), list(range(n_m, 2 * n_m))
+    if all_m:
+        cap_set = lat_set = list(range(n_m))
+    else:
+        cap_set, lat_set = list(range(n_m)), list(range(n_m, 2 * n_m))
 
     def run(idx, arrivals):
         rows = {}
@@ -303,7 +352,7 @@ def bench_stream(engine, pgap, n_m, load, seed, cpu_sample, threads, P4=1511, D4
         r0 = rows_cap[i][0]
         assert (got[0].prev_p, got[0].next_p) == (r0.prev_p, r0.next_p)
     out = {
-        "workload": f"P={P4} d={D4} SL={SL4}, m in seeded random order from [1,{m_hi}) coprime to d, unknowns "
+        "workload": f"P={P4} d={D4} SL={SL4}, {'every' if all_m else ''} m in seeded random order from [1,{m_hi}) coprime to d, unknowns "
                     f"sieved to {sieve_limit:.0e} ({np.mean([len(p) + len(q) for p, q in lists]):.0f} per m), Fermat-2 scan "
                     f"with Baillie-PSW confirmation of each side's first PRP (strong base 2 + strong Lucas), per-round depth sized to the resident lanes (cap 256)",
         "m": n_m,
@@ -332,8 +381,10 @@ def bench_stream(engine, pgap, n_m, load, seed, cpu_sample, threads, P4=1511, D4
         out["cpu_baseline"] = {"m_per_s": len(ids) / dt, "tests_per_s": float(rows[:, 2].sum()) / dt,
                                "cores": threads, "kind": "port",
                                "sample": f"{len(ids)} of the same m's, gap_test_simple semantics "
-                                         f"(mpz_probab_prime_p(N,25) scan), {threads} threads, {dt:.1f} s"}
+                                         f"(mpz_probab_prime_p(N,25) scan), {threads} threads, {dt:.1f} s",
+                               "host": cpu_info}
         out["parity_sample_mismatches"] = mism
+        out["parity_sample_m"] = len(ids)
     return out
 
 
@@ -354,6 +405,8 @@ def main():
     ap.add_argument("--no-stream-4300", action="store_true", help="skip the large-operand serving measurement")
     ap.add_argument("--stream-4300-m", type=int, default=2048)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-width batch")
+    ap.add_argument("--no-plain-503", action="store_true", help="skip the plain 503#/210 file scan")
+    ap.add_argument("--mad-peak", action="store_true", help="only measure the v_mad_u64_u32 rate")
     ap.add_argument("--no-4300", action="store_true", help="skip the 4300-bit batch")
     ap.add_argument("--stream-m", type=int, default=16384)
     ap.add_argument("--stream-load", type=float, default=0.5)
@@ -382,6 +435,13 @@ def main():
     import pgap
 
     engine = pgap.Engine(0)   # raises PgapError when no GPU / no HIP code object
+    if args.mad_peak:
+        pk = measured_peak(engine)
+        print(json.dumps({"mad_peak": pk, "nominal_mac_per_s": PEAK_INT32_MAC,
+                          "frac_of_nominal": pk["mac_per_s"] / PEAK_INT32_MAC}), flush=True)
+        engine.close()
+        return 0
+    peak = measured_peak(engine)
     fam = engine.family(P, D, 0)
     t_setup = time.perf_counter()
     m, x = draw_candidates(engine, fam, args.count, args.seed)
@@ -457,12 +517,18 @@ def main():
         "roofline": {
             "bound": "valu",
             "achieved": achieved / 1e12,
-            "peak": PEAK_INT32_MAC / 1e12,
+            "peak": peak["mac_per_s"] / 1e12,
             "unit": "TOP/s (int32 multiply-add)",
-            "frac": achieved / PEAK_INT32_MAC,
+            "frac": achieved / peak["mac_per_s"],
             "traffic": load_traffic(f"fermat2_r{info.radix_bits}_d{info.digits}"),
-            "work_per_test": f"2*L^2*e int32 MACs, L={L32} limbs, e={e_mean:.1f}",
-            "frac_container_40_limbs": w_container * args.count / avg_kernel_s / PEAK_INT32_MAC,
+            "peak_source": f"measured in this run: pgap_mad_peak, {peak['macs_per_launch']} v_mad_u64_u32 in "
+                           f"{peak['kernel_ms']:.2f} ms (best of 3)",
+            "peak_nominal": PEAK_INT32_MAC / 1e12,
+            "frac_of_nominal": achieved / PEAK_INT32_MAC,
+            "work_per_test": f"2*L^2*e int32 MACs, L={L32} limbs (minimal), e={e_mean:.1f}",
+            "work_per_launch": w_test * args.count,
+            "frac_container_40_limbs": w_container * args.count / avg_kernel_s / peak["mac_per_s"],
+            "frac_container_40_limbs_of_nominal": w_container * args.count / avg_kernel_s / PEAK_INT32_MAC,
             "kernel_ms_avg": avg_kernel_s * 1e3,
         },
         "prp_found": int(verd.sum()),
@@ -470,8 +536,8 @@ def main():
         "host_buffer_verdicts_identical": bool((host_verd == verd).all()),
         "setup_s": round(setup_s, 2),
     }
+    threads, cpu_info = host_cpus()
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
         host = d_n.to_host(rows=args.cpu_sample)
         rate, dt, cv = cpu_baseline(host, args.cpu_sample, threads)
         mism = int((cv != verd[: args.cpu_sample]).sum())
@@ -482,27 +548,35 @@ def main():
             "kind": "port",
             "sample": f"first {args.cpu_sample} of the same candidates, Fermat base 2 via GMP mpz_powm, "
                       f"{threads} threads, {dt:.1f} s",
+            "host": cpu_info,
         }
         result["parity_sample_mismatches"] = mism
     extra = {}
     if not args.no_2180:
-        extra["batch_2180_bit"] = bench_batch_2180(engine, hipmem, pgap, 1 << 18, 3, args.seed + 1)
+        extra["batch_2180_bit"] = bench_batch_2180(engine, hipmem, pgap, 1 << 18, 3, args.seed + 1, peak["mac_per_s"])
     if not args.no_mixed:
         extra["mixed_widths"] = bench_mixed(engine, hipmem, pgap, 1 << 19, args.seed + 3)
     if not args.no_4300:
-        extra["batch_4300_bit"] = bench_batch_4300(engine, hipmem, pgap, 1 << 15, args.seed + 4)
+        extra["batch_4300_bit"] = bench_batch_4300(engine, hipmem, pgap, 1 << 15, args.seed + 4, peak["mac_per_s"])
+    if not args.no_plain_503:
+        # the reference's CPU-runnable case: 503#/210, SL=4000, every m < 2000
+        # coprime to 210, unknowns sieved to 1e8 (a fresh engine: a scan's
+        # slots are sized by the families it starts with)
+        engine3 = pgap.Engine(0)
+        extra["plain_503"] = bench_stream(engine3, pgap, 0, args.stream_load, args.seed + 6,
+                                          0 if args.no_cpu else 10 ** 9, threads, P4=503, D4=210, SL4=4000,
+                                          m_hi=2000, sieve_limit=100_000_000, all_m=True, cpu_info=cpu_info)
+        engine3.close()
     if not args.no_stream:
-        threads = min(16, os.cpu_count() or 1)
         extra["stream"] = bench_stream(engine, pgap, args.stream_m, args.stream_load, args.seed + 2,
-                                       0 if args.no_cpu else args.stream_cpu_sample, threads)
+                                       0 if args.no_cpu else args.stream_cpu_sample, threads, cpu_info=cpu_info)
     if not args.no_stream_4300:
         # P=3001, d=2190, SL=40000, m < 1e5, unknowns sieved to 1e10
         # (a fresh engine: a scan's slots are sized by the families it starts with)
         engine2 = pgap.Engine(0)
-        threads = min(16, os.cpu_count() or 1)
         extra["stream_4300_bit"] = bench_stream(engine2, pgap, args.stream_4300_m, args.stream_load, args.seed + 5,
                                                 0 if args.no_cpu else 16, threads, P4=3001, D4=2190, SL4=40000,
-                                                m_hi=100000, sieve_limit=10_000_000_000)
+                                                m_hi=100000, sieve_limit=10_000_000_000, cpu_info=cpu_info)
         engine2.close()
     if extra:
         result["extra"] = extra

@@ -229,6 +229,14 @@ pgap_status pgap_scan_stats(const pgap_engine *e, uint64_t *tests_issued, uint64
  * THREADS_PER_INSTANCE (gap_test_gpu.cu:48-81). */
 pgap_status pgap_autotune(pgap_engine *e, const uint32_t *bits, size_t n, const char *path);
 
+/* The denominator of the PRP kernels' roofline, measured: `iters` rounds of
+ * independent v_mad_u64_u32 chains (32x32-bit products accumulated into 64
+ * bits, the instruction the Montgomery squarings issue) on every compute
+ * unit at 8 waves per SIMD.  Reports the sustained multiply-add rate, the
+ * timed launch's kernel time and its multiply-add count.  No counterpart in
+ * the reference (measurement only). */
+pgap_status pgap_mad_peak(pgap_engine *e, uint32_t iters, double *mac_per_s, double *kernel_ms, uint64_t *macs);
+
 /* Side-skip, gap_test_gpu's order (gap_test_gpu.cu:591-617): each m scans
  * its prev side first and starts the next side only after the prev PRP; when
  * that PRP's merit, (float)(prev_p / (K_log + ln m)), is below

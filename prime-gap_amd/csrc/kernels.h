@@ -66,7 +66,7 @@ struct Variant {
     int lanes;       // lanes cooperating on one number
     int kind;        // Kind
     int max_bits;    // largest N the variant accepts: Fermat N < R/16; strong PRP N < R/2^8,
-                     // which covers bases up to 8 (see select_bits); strong Lucas N < R/4
+                     // which covers bases up to 8 (see select_bits); strong Lucas N < R/2^54
     batch_launch_fn launch;
     int (*blocks_per_cu)();  // resident 256-thread workgroups per CU (occupancy query)
     scan_launch_fn scan = nullptr;  // set on strong-PRP variants (max_bits is the SPRP limit)

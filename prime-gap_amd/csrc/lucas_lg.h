@@ -11,15 +11,21 @@
 //   N passes if U_d = 0 or V_{d*2^r} = 0 (mod N) for some 0 <= r < s.
 // 2 passes; other even N, N < 2 and perfect squares fail.
 //
-// Arithmetic: every value is held fully reduced (exact digits, < N) in
-// Montgomery form, so the products use mul_redc / sqr_redc followed by one
-// conditional subtraction, and the additive steps are modular.  The chain
-// runs on (V_k, V_k+1, Q^k) with branch-free selects (numbers of one
+// Arithmetic (Montgomery form, exact 28-bit digits throughout).  The chain
+// runs on (V_k, V_k+1, Q^k) with branch-free selects (the numbers of one
 // wavefront take different exponent bits):
-//   bit 0:  V_2k   = V_k^2 - 2 Q^k,         V_2k+1 = V_k V_k+1 - Q^k,   Q^2k   = (Q^k)^2
-//   bit 1:  V_2k+1 = V_k V_k+1 - Q^k,       V_2k+2 = V_k+1^2 - 2 Q^k+1, Q^2k+1 = Q^k Q^k+1
-// i.e. four Montgomery products per exponent bit.  U_d is never formed:
-// D U_d = 2 V_d+1 - P V_d and gcd(D, N) = 1, so U_d = 0 iff 2 V_d+1 = V_d.
+//   bit 0:  V_2k   = V_k^2 - 2 Q^k,      V_2k+1 = V_k V_k+1 - Q^k,   Q^2k   = Q^k (Q^k)
+//   bit 1:  V_2k+1 = V_k V_k+1 - Q^k,    V_2k+2 = V_k+1^2 - 2 Q^k+1, Q^2k+1 = Q^k (Q Q^k)
+// Q is small (|Q| < 2^23), so Q^(k+bit) = c Q^k with c = 1 or Q is a scalar
+// multiple that is never reduced: Q^k stays below N, c Q^k below 2^24 N, and
+// the V values are kept below 2^26 N by adding 2^25 N before subtracting
+// 2 c Q^k.  With N < R / 2^54 (the variants' width bound) a Montgomery
+// product of two such values is below 2N, so each exponent bit costs three
+// products (V_k V_k+1, X^2, Q^k (c Q^k)) and linear work, and no V is ever
+// reduced.  The final tests read residues through one Montgomery reduction
+// of the value itself ((v + mN)/R lies in [0, N] for v < R: zero iff v = 0
+// mod N).  U_d is never formed: D U_d = 2 V_d+1 - P V_d and gcd(D, N) = 1,
+// so U_d = 0 iff 2 V_d+1 = V_d.
 #pragma once
 #include "prp_lg.h"
 
@@ -89,6 +95,8 @@ __device__ bool is_square(const uint32_t (&n)[S], uint32_t b, uint32_t g) {
 // suitable D under GRH (the least prime q with Jacobi(N/q) = -1 is below
 // 2 ln(4N)^2, 2.0e7 at 4480 bits); an N reaching it is reported composite.
 constexpr uint32_t kLucasMaxD = 1u << 25;
+// Bits of headroom below R = 2^(28 G S) the chain needs: N < R / 2^54.
+constexpr int kLucasHeadroom = 54;
 // Candidates tried before the perfect-square check (a square has no D).
 constexpr uint32_t kLucasSquareCheck = 32;
 
@@ -119,33 +127,19 @@ __device__ __forceinline__ uint32_t strong_lucas(const uint32_t *__restrict__ nw
     }
     if (D == 0) return 0;
     const int32_t Q = (1 - D) / 4;
+    const uint32_t qa = Q < 0 ? (uint32_t)(-Q) : (uint32_t)Q;
 
     const uint32_t np = neg_inv(n0);
-    uint32_t one[S], vk[S];
-    mont_one_two<S, G>(n, b, g, one, vk);  // V_0 = 2
-    uint32_t qm[S];                        // Mont(Q)
+    uint32_t vk[S], vk1[S], qk[S];
+    mont_one_two<S, G>(n, b, g, vk1, vk);  // V_1 = P = 1, V_0 = 2
+#pragma unroll
+    for (int p = 0; p < S; ++p) qk[p] = vk1[p];  // Q^0 = 1
+    uint32_t w25[S];  // 2^25 N
     {
-        const uint32_t qa = Q < 0 ? (uint32_t)(-Q) : (uint32_t)Q;
+        uint64_t t[S];
 #pragma unroll
-        for (int p = 0; p < S; ++p) qm[p] = 0;
-        for (int t = 31 - __builtin_clz(qa); t >= 0; --t) {
-            mod_double<S, G>(qm, n, g);
-            if ((qa >> t) & 1u) mod_add<S, G>(qm, one, n, g);
-        }
-        if (Q < 0) {
-            uint32_t z[S];
-#pragma unroll
-            for (int p = 0; p < S; ++p) z[p] = 0;
-            mod_sub<S, G>(z, qm, n, g);
-#pragma unroll
-            for (int p = 0; p < S; ++p) qm[p] = z[p];
-        }
-    }
-    uint32_t vk1[S], qk[S];  // V_1 = P = 1, Q^0 = 1
-#pragma unroll
-    for (int p = 0; p < S; ++p) {
-        vk1[p] = one[p];
-        qk[p] = one[p];
+        for (int p = 0; p < S; ++p) t[p] = (uint64_t)n[p] << 25;
+        normalize<S, G>(t, w25, g);
     }
 
     // s = trailing ones of N; the bits of N + 1 from the top down to s are
@@ -155,29 +149,46 @@ __device__ __forceinline__ uint32_t strong_lucas(const uint32_t *__restrict__ nw
     const int top = s == b ? (int)b : (int)b - 1;
     for (int i = top; i >= (int)s; --i) {
         const uint32_t bit = i == (int)s ? 1u : ((nw[i >> 5] >> (i & 31)) & 1u);
-        uint32_t x[S], p1[S], qx[S], q2[S];
+        const uint32_t c = bit ? qa : 1u;
+        const bool neg = bit && Q < 0;  // Q^(k+bit) = -c Q^k
+        uint32_t x[S], p1[S], qc[S];
+        uint64_t t[S];
 #pragma unroll
         for (int p = 0; p < S; ++p) {
             x[p] = bit ? vk1[p] : vk[p];
             p1[p] = vk[p];
-            qx[p] = qk[p];
+            t[p] = (uint64_t)qk[p] * c;
         }
-        mul_redc<S, G>(p1, vk1, n, np, g, stage);  // V_k V_k+1 - Q^k
-        cond_sub<S, G>(p1, n, g);
-        mod_sub<S, G>(p1, qk, n, g);
-        mul_redc<S, G>(qx, qm, n, np, g, stage);   // Q^k+1 when the bit is set
-        cond_sub<S, G>(qx, n, g);
+        normalize<S, G>(t, qc, g);                 // c Q^k < 2^24 N
+        // P1 = V_k V_k+1 - Q^k, as (V_k V_k+1) + N - Q^k in (0, 3N)
+        mul_redc<S, G>(p1, vk1, n, np, g, stage);
 #pragma unroll
-        for (int p = 0; p < S; ++p) {
-            qx[p] = bit ? qx[p] : qk[p];
-            q2[p] = qx[p];
+        for (int p = 0; p < S; ++p) t[p] = (uint64_t)p1[p] + n[p];
+        normalize<S, G>(t, p1, g);
+        (void)sub_n<S, G>(p1, qk, p1, g);
+        // P2 = X^2 - 2 Q^(k+bit), as X^2 + 2^25 N - 2 c Q^k (or X^2 + 2 c Q^k)
+        sqr_redc<S, G>(x, n, np, g, stage);
+        {
+            uint32_t z[S], a[S];
+#pragma unroll
+            for (int p = 0; p < S; ++p) z[p] = qc[p];
+            shl<S, G>(z, 1u, g);                   // 2 c Q^k < 2^25 N
+            (void)sub_n<S, G>(w25, z, a, g);       // 2^25 N - 2 c Q^k >= 0
+#pragma unroll
+            for (int p = 0; p < S; ++p) t[p] = (uint64_t)x[p] + (neg ? z[p] : a[p]);
+            normalize<S, G>(t, x, g);              // < 2N + 2^25 N < 2^26 N
         }
-        sqr_redc<S, G>(x, n, np, g, stage);        // X^2 - 2 Q^(k+bit)
-        cond_sub<S, G>(x, n, g);
-        mod_double<S, G>(q2, n, g);
-        mod_sub<S, G>(x, q2, n, g);
-        mul_redc<S, G>(qk, qx, n, np, g, stage);   // Q^(2k+bit)
+        // Q^(2k+bit) = Q^k (c Q^k), sign applied
+        mul_redc<S, G>(qk, qc, n, np, g, stage);
         cond_sub<S, G>(qk, n, g);
+        if (neg) {
+            uint32_t z[S];
+#pragma unroll
+            for (int p = 0; p < S; ++p) z[p] = 0;
+            mod_sub<S, G>(z, qk, n, g);
+#pragma unroll
+            for (int p = 0; p < S; ++p) qk[p] = z[p];
+        }
 #pragma unroll
         for (int p = 0; p < S; ++p) {
             vk[p] = bit ? p1[p] : x[p];
@@ -185,21 +196,43 @@ __device__ __forceinline__ uint32_t strong_lucas(const uint32_t *__restrict__ nw
         }
     }
 
-    // U_d = 0 iff 2 V_d+1 = V_d; then V_d, V_2d, ..., V_{d 2^(s-1)}
-    uint32_t w[S];
+    // U_d = 0 iff 2 V_d+1 - V_d = 0; then V_d, V_2d, ..., V_{d 2^(s-1)}
+    uint32_t e1[S];  // the integer 1: a Montgomery product with it reduces
 #pragma unroll
-    for (int p = 0; p < S; ++p) w[p] = vk1[p];
-    mod_double<S, G>(w, n, g);
-    if (is_zero<S, G>(vk) || equal<S, G>(w, vk)) return 1;
+    for (int p = 0; p < S; ++p) e1[p] = (g == 0 && p == 0) ? 1u : 0u;
+    const auto zero_mod_n = [&](const uint32_t (&v)[S]) {
+        uint32_t r[S];
+#pragma unroll
+        for (int p = 0; p < S; ++p) r[p] = v[p];
+        mul_redc<S, G>(r, e1, n, np, g, stage);  // v R^-1 mod N, in [0, N]
+        return is_zero<S, G>(r) || equal<S, G>(r, n);
+    };
+    if (zero_mod_n(vk)) return 1;
+    {
+        uint32_t w[S], a[S];
+        uint64_t t[S];
+#pragma unroll
+        for (int p = 0; p < S; ++p) w[p] = w25[p];
+        shl<S, G>(w, 1u, g);                       // 2^26 N > V_d
+        (void)sub_n<S, G>(w, vk, a, g);
+#pragma unroll
+        for (int p = 0; p < S; ++p) t[p] = ((uint64_t)vk1[p] << 1) + a[p];
+        normalize<S, G>(t, w, g);                  // 2 V_d+1 + 2^26 N - V_d < 2^28 N
+        if (zero_mod_n(w)) return 1;
+    }
     for (uint32_t r = 1; r < s; ++r) {
-        uint32_t q2[S];
+        // V <- V^2 - 2 Q^k, Q^k <- (Q^k)^2
+        uint32_t z[S], a[S];
+        uint64_t t[S];
 #pragma unroll
-        for (int p = 0; p < S; ++p) q2[p] = qk[p];
-        mod_double<S, G>(q2, n, g);
+        for (int p = 0; p < S; ++p) z[p] = qk[p];
+        shl<S, G>(z, 1u, g);
+        (void)sub_n<S, G>(w25, z, a, g);
         sqr_redc<S, G>(vk, n, np, g, stage);
-        cond_sub<S, G>(vk, n, g);
-        mod_sub<S, G>(vk, q2, n, g);
-        if (is_zero<S, G>(vk)) return 1;
+#pragma unroll
+        for (int p = 0; p < S; ++p) t[p] = (uint64_t)vk[p] + a[p];
+        normalize<S, G>(t, vk, g);
+        if (zero_mod_n(vk)) return 1;
         sqr_redc<S, G>(qk, n, np, g, stage);
         cond_sub<S, G>(qk, n, g);
     }

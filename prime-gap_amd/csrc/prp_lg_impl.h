@@ -140,7 +140,8 @@ struct RegisterLG {
 template <int G, int... Ss>
 struct RegisterLucas {
     RegisterLucas() {
-        (register_variant(Variant{lg::RB, G * Ss, G, KIND_LUCAS, lg::RB * G * Ss - 2, &launch_lg<Ss, G, KIND_LUCAS>,
+        (register_variant(Variant{lg::RB, G * Ss, G, KIND_LUCAS, lg::RB * G * Ss - lg::kLucasHeadroom,
+                                  &launch_lg<Ss, G, KIND_LUCAS>,
                                   &occupancy_lg<Ss, G, KIND_LUCAS>}),
          ...);
     }

@@ -20,7 +20,8 @@
 //   exhausted sides.
 // Strong-PRP confirmations ride in the next round's grid instead of a launch
 // of their own, so a round costs one test's latency; strong Lucas rows (lane
-// groups only) follow in a launch of their own on the same stream.  A hit
+// groups only) run in a launch of their own on a second stream, beside the
+// grid, and are joined before the retire step.  A hit
 // that fails a confirmation (a base-2 pseudoprime) restarts its side just
 // past it.
 // Every index below a side's first PRP has been tested in this or an earlier
@@ -438,6 +439,10 @@ class Scanner {
     bool stop = false;
     std::thread worker;
     hipStream_t stream = nullptr;      // the worker's stream
+    // the strong Lucas rows run beside the round's grid on a stream of their
+    // own (default priority), joined before the retire step
+    hipStream_t lstream = nullptr;
+    hipEvent_t ev_rows = nullptr, ev_lucas = nullptr;
 
     std::vector<SlotHost> slots;
     std::vector<uint32_t> free_slots, active;
@@ -497,7 +502,13 @@ class Scanner {
         h_active = h_ctr = h_updata = nullptr;
         h_events = nullptr;
         h_up = nullptr;
+        if (lstream) (void)hipStreamSynchronize(lstream);
+        if (ev_rows) (void)hipEventDestroy(ev_rows);
+        if (ev_lucas) (void)hipEventDestroy(ev_lucas);
+        if (lstream) (void)hipStreamDestroy(lstream);
         if (stream) (void)hipStreamDestroy(stream);
+        ev_rows = ev_lucas = nullptr;
+        lstream = nullptr;
         stream = nullptr;
         inited = false;
     }
@@ -613,10 +624,13 @@ class Scanner {
             release();
             return st;
         }
-        hipError_t he = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
-        if (he != hipSuccess) {
+        hipError_t he;
+        if ((he = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess ||
+            (he = hipStreamCreateWithFlags(&lstream, hipStreamNonBlocking)) != hipSuccess ||
+            (he = hipEventCreateWithFlags(&ev_rows, hipEventDisableTiming)) != hipSuccess ||
+            (he = hipEventCreateWithFlags(&ev_lucas, hipEventDisableTiming)) != hipSuccess) {
             release();
-            return e->hip_fail(he, "scan stream");
+            return e->hip_fail(he, "scan streams");
         }
         slots.assign(S, SlotHost());
         free_slots.clear();
@@ -690,23 +704,27 @@ class Scanner {
         uint64_t end = (uint64_t)x + FALLBACK_SPAN;
         if (side == 0 && sh.small) end = std::min<uint64_t>(end, sh.small - 2);  // x <= centre - 3
         *end_of_range = side == 0 && sh.small && x >= end;
-        std::vector<uint32_t> mr(fb_primes.size());
-        for (size_t i = 0; i < fb_primes.size(); ++i)
-            mr[i] = (uint32_t)((sh.m % fb_primes[i]) * km[i] % fb_primes[i]);
-        for (; x < end && out.size() < FALLBACK_CHUNK; ++x) {
-            // N = m*K - x (prev) or m*K + x (next), exactly when the centre is small
-            const uint64_t N = sh.small ? (side ? sh.small + x : sh.small - x) : ~0ull;
-            bool ok = true;
-            for (size_t i = 0; i < fb_primes.size() && ok; ++i) {
-                const uint32_t p = fb_primes[i];
-                const uint32_t xr = x % p;
-                // divisible iff m*K == +-x mod p
-                const bool div = side ? ((mr[i] + xr) % p == 0) : (mr[i] == xr);
-                ok = !div || N == p;
+        if (x >= end) return out;
+        // sieve [x, end): N = m*K - x (prev) or m*K + x (next) is divisible by p
+        // iff x == m*K (prev) or x == -m*K (next) mod p; N itself a small
+        // prime (small centres only) stays
+        const uint32_t span = (uint32_t)(end - x);
+        std::vector<uint8_t> comp(span, 0);
+        for (size_t i = 0; i < fb_primes.size(); ++i) {
+            const uint32_t p = fb_primes[i];
+            const uint32_t r = (uint32_t)((sh.m % p) * km[i] % p);
+            const uint32_t hit = side ? (p - r) % p : r;                  // x mod p of the multiples
+            uint64_t j = (uint64_t)((hit + p - x % p) % p);               // first index with x + j == hit
+            for (; j < span; j += p) {
+                const uint64_t xx = x + j;
+                const uint64_t N = sh.small ? (side ? sh.small + xx : sh.small - xx) : ~0ull;
+                if (N != p) comp[j] = 1;
             }
-            if (ok) out.push_back(x);
         }
-        sh.fb_from[side] = x;
+        uint32_t j = 0;
+        for (; j < span && out.size() < FALLBACK_CHUNK; ++j)
+            if (!comp[j]) out.push_back(x + j);
+        sh.fb_from[side] = x + j;
         return out;
     }
 
@@ -878,18 +896,26 @@ class Scanner {
         hipLaunchKernelGGL(materialize_kernel, dim3((nrows + 255) / 256), b256, 0, stream, d_items, d_idx, cr, lr0,
                            nrows, d_ctr, d_center, CW, d_lists, CAP, d_nbuf, W);
         if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "scan issue");
+        if (lucas) {
+            // the round's strong Lucas rows (count on the device in ctr[6]),
+            // concurrent with the grid below: they need only the materialised rows
+            if ((he = hipEventRecord(ev_rows, stream)) != hipSuccess ||
+                (he = hipStreamWaitEvent(lstream, ev_rows, 0)) != hipSuccess)
+                return e->hip_fail(he, "scan Lucas stream wait");
+            he = cl.v->launch(d_nbuf + (size_t)lr0 * W, W, nl, d_ctr + 6, nullptr, none, d_verdict + lr0, cl.cap,
+                              lstream);
+            if (he != hipSuccess) return e->hip_fail(he, "scan strong Lucas launch");
+            if ((he = hipEventRecord(ev_lucas, lstream)) != hipSuccess)
+                return e->hip_fail(he, "scan Lucas event");
+        }
         if (vc)
             he = vc->scan(d_nbuf, W, cr, cr, maxi, d_ctr, d_base, d_verdict, cap_c, stream);
         else
             he = vf->launch(d_nbuf + (size_t)cr * W, W, maxi, d_ctr, nullptr, none, d_verdict + cr, cap_f,
                             stream);
         if (he != hipSuccess) return e->hip_fail(he, "scan test launch");
-        if (lucas) {
-            // the round's strong Lucas rows (count on the device in ctr[6])
-            he = cl.v->launch(d_nbuf + (size_t)lr0 * W, W, nl, d_ctr + 6, nullptr, none, d_verdict + lr0, cl.cap,
-                              stream);
-            if (he != hipSuccess) return e->hip_fail(he, "scan strong Lucas launch");
-        }
+        if (lucas && (he = hipStreamWaitEvent(stream, ev_lucas, 0)) != hipSuccess)
+            return e->hip_fail(he, "scan Lucas join");
         hipLaunchKernelGGL(retire_kernel, dim3((nrows + 255) / 256), b256, 0, stream, d_items, d_idx, cr, lr0,
                            nrows, d_verdict, d_ctr, confirm, d_state);
         hipLaunchKernelGGL(update_kernel, dim3((na + 255) / 256), b256, 0, stream, d_active, na, d_state, d_lists,

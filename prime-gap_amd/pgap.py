@@ -57,7 +57,7 @@ EXPORTED = (
     "pgap_last_launch", "pgap_family", "pgap_scan_submit", "pgap_scan_poll",
     "pgap_scan_in_flight", "pgap_scan_config", "pgap_build_candidates_device", "pgap_filter_candidates",
     "pgap_sieve", "pgap_scan_stats", "pgap_autotune", "pgap_scan_side_skip",
-    "pgap_prp_batch_device_noalloc",
+    "pgap_prp_batch_device_noalloc", "pgap_mad_peak",
 )
 NOALLOC_MAX_COUNT = 1 << 20
 
@@ -155,6 +155,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.pgap_sieve.argtypes = [vp, ctypes.c_int, vp, sz, u64, vp, vp, vp, vp]
     lib.pgap_scan_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u32)]
     lib.pgap_autotune.argtypes = [vp, ctypes.POINTER(u32), ctypes.c_size_t, ctypes.c_char_p]
+    lib.pgap_mad_peak.argtypes = [vp, u32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(u64)]
     for name in EXPORTED:
         f = getattr(lib, name)
         if name not in ("pgap_destroy", "pgap_last_error", "pgap_version", "pgap_scan_in_flight"):
@@ -363,6 +365,12 @@ class Engine:
         finally:
             if tmp:
                 os.unlink(tmp)
+
+    def mad_peak(self, iters: int = 4096) -> dict:
+        """Sustained v_mad_u64_u32 rate of the device (the roofline denominator)."""
+        rate, ms, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_uint64()
+        self._check(self.lib.pgap_mad_peak(self.h, iters, ctypes.byref(rate), ctypes.byref(ms), ctypes.byref(n)))
+        return {"mac_per_s": rate.value, "kernel_ms": ms.value, "macs": n.value}
 
     def scan_stats(self) -> dict:
         t, r, s = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint32()

@@ -26,7 +26,7 @@ def test_autotune_table_and_parity(engine, tmp_path):
         if r["test"] == pgap.LUCAS:
             assert r["lanes"] in (8, 16)
         # the chosen variant covers the width (strong PRP tuned with base 3)
-        limit = r["radix"] * r["digits"] - {pgap.FERMAT2: 4, pgap.SPRP: 8, pgap.LUCAS: 2}[r["test"]]
+        limit = r["radix"] * r["digits"] - {pgap.FERMAT2: 4, pgap.SPRP: 8, pgap.LUCAS: 54}[r["test"]]
         assert limit >= r["bits"], r
 
     tuned = pgap.Engine(0, tuning_path=path)
