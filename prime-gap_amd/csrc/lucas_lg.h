@@ -26,6 +26,15 @@
 // of the value itself ((v + mN)/R lies in [0, N] for v < R: zero iff v = 0
 // mod N).  U_d is never formed: D U_d = 2 V_d+1 - P V_d and gcd(D, N) = 1,
 // so U_d = 0 iff 2 V_d+1 = V_d.
+//
+// The three products of a step depend only on the state before it.  With
+// WAVE set, one number takes a whole wavefront: its four 16-lane DPP rows
+// each hold the full state, rows 0, 1 and 2 compute V_k V_k+1, X^2 and
+// Q^k (c Q^k) in one Montgomery product with per-row operands (row 3
+// repeats X^2), and the rows trade their results through LDS (`xch`: 64 S
+// words per wavefront, in order within the wavefront, so no barrier).  A
+// step then costs one product's latency instead of three, for the strong
+// Lucas tests a streaming scan waits on.
 #pragma once
 #include "prp_lg.h"
 
@@ -100,9 +109,48 @@ constexpr int kLucasHeadroom = 54;
 // Candidates tried before the perfect-square check (a square has no D).
 constexpr uint32_t kLucasSquareCheck = 32;
 
-template <int S, int G>
+// p1 = V_k V_k+1, x2 = X^2, q3 = Q^k (c Q^k), each R^-1 mod N and below 2N.
+template <int S, int G, bool WAVE>
+__device__ __forceinline__ void lucas_products(const uint32_t (&vk)[S], const uint32_t (&vk1)[S],
+                                               const uint32_t (&x)[S], const uint32_t (&qk)[S],
+                                               const uint32_t (&qc)[S], uint32_t (&p1)[S], uint32_t (&x2)[S],
+                                               uint32_t (&q3)[S], const uint32_t (&n)[S], uint32_t np, uint32_t g,
+                                               uint32_t row, uint32_t *stage, uint32_t *xch) {
+    if constexpr (WAVE) {
+        uint32_t a[S], y[S];
+#pragma unroll
+        for (int p = 0; p < S; ++p) {
+            a[p] = row == 0 ? vk[p] : (row == 2 ? qk[p] : x[p]);
+            y[p] = row == 0 ? vk1[p] : (row == 2 ? qc[p] : x[p]);
+        }
+        mul_redc<S, G>(a, y, n, np, g, stage);
+#pragma unroll
+        for (int p = 0; p < S; ++p) xch[(row * S + p) * 16 + g] = a[p];
+#pragma unroll
+        for (int p = 0; p < S; ++p) {
+            p1[p] = xch[(0 * S + p) * 16 + g];
+            x2[p] = xch[(1 * S + p) * 16 + g];
+            q3[p] = xch[(2 * S + p) * 16 + g];
+        }
+    } else {
+#pragma unroll
+        for (int p = 0; p < S; ++p) {
+            p1[p] = vk[p];
+            x2[p] = x[p];
+            q3[p] = qk[p];
+        }
+        mul_redc<S, G>(p1, vk1, n, np, g, stage);
+        sqr_redc<S, G>(x2, n, np, g, stage);
+        mul_redc<S, G>(q3, qc, n, np, g, stage);
+    }
+}
+
+// WAVE: `row` is the lane's 16-lane row of its wavefront and `xch` the
+// wavefront's exchange area (G must be 16).
+template <int S, int G, bool WAVE = false>
 __device__ __forceinline__ uint32_t strong_lucas(const uint32_t *__restrict__ nw, uint32_t limbs, uint32_t g,
-                                                 uint32_t *stage) {
+                                                 uint32_t *stage, uint32_t row = 0, uint32_t *xch = nullptr) {
+    static_assert(!WAVE || G == 16, "a wavefront's rows are 16-lane groups");
     uint32_t n[S];
     load_digits<S, G>(nw, limbs, g, n);
     const uint32_t b = bitlen<S, G>(n, g);
@@ -151,23 +199,21 @@ __device__ __forceinline__ uint32_t strong_lucas(const uint32_t *__restrict__ nw
         const uint32_t bit = i == (int)s ? 1u : ((nw[i >> 5] >> (i & 31)) & 1u);
         const uint32_t c = bit ? qa : 1u;
         const bool neg = bit && Q < 0;  // Q^(k+bit) = -c Q^k
-        uint32_t x[S], p1[S], qc[S];
+        uint32_t x[S], p1[S], qc[S], q3[S];
         uint64_t t[S];
 #pragma unroll
         for (int p = 0; p < S; ++p) {
             x[p] = bit ? vk1[p] : vk[p];
-            p1[p] = vk[p];
             t[p] = (uint64_t)qk[p] * c;
         }
         normalize<S, G>(t, qc, g);                 // c Q^k < 2^24 N
+        lucas_products<S, G, WAVE>(vk, vk1, x, qk, qc, p1, x, q3, n, np, g, row, stage, xch);
         // P1 = V_k V_k+1 - Q^k, as (V_k V_k+1) + N - Q^k in (0, 3N)
-        mul_redc<S, G>(p1, vk1, n, np, g, stage);
 #pragma unroll
         for (int p = 0; p < S; ++p) t[p] = (uint64_t)p1[p] + n[p];
         normalize<S, G>(t, p1, g);
         (void)sub_n<S, G>(p1, qk, p1, g);
         // P2 = X^2 - 2 Q^(k+bit), as X^2 + 2^25 N - 2 c Q^k (or X^2 + 2 c Q^k)
-        sqr_redc<S, G>(x, n, np, g, stage);
         {
             uint32_t z[S], a[S];
 #pragma unroll
@@ -179,7 +225,8 @@ __device__ __forceinline__ uint32_t strong_lucas(const uint32_t *__restrict__ nw
             normalize<S, G>(t, x, g);              // < 2N + 2^25 N < 2^26 N
         }
         // Q^(2k+bit) = Q^k (c Q^k), sign applied
-        mul_redc<S, G>(qk, qc, n, np, g, stage);
+#pragma unroll
+        for (int p = 0; p < S; ++p) qk[p] = q3[p];
         cond_sub<S, G>(qk, n, g);
         if (neg) {
             uint32_t z[S];

@@ -113,10 +113,41 @@ __device__ __forceinline__ uint32_t neg_inv(uint32_t n0) {
     return (0u - x) & MASK;
 }
 
+// lane g <- lane g - SH of the same 16-lane DPP row (lanes below SH read 0).
+template <int SH>
+__device__ __forceinline__ uint32_t row_shr(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 + SH, 0xF, 0xF, true);
+}
+
+// One-bit carries (or borrows) across the group by carry lookahead: lane g
+// generates `gen` (a carry out of its digits) and propagates (`prop`: a
+// carry into it would pass through); a Kogge-Stone scan over the lanes
+// (log2 G DPP steps instead of G - 1 rounds over S digits) gives each lane
+// its carry in.
+template <int G, int SH>
+__device__ __forceinline__ void lookahead_step(uint32_t &gp, uint32_t g) {
+    if constexpr (SH < G) {
+        uint32_t b = row_shr<SH>(gp);
+        if (g < (uint32_t)SH) b = 2u;  // lanes of the group below lane 0: gen 0, prop 1
+        gp = ((gp | ((gp >> 1) & b)) & 1u) | (gp & b & 2u);
+        lookahead_step<G, SH * 2>(gp, g);
+    }
+}
+template <int G>
+__device__ __forceinline__ uint32_t carry_in(uint32_t gen, uint32_t prop, uint32_t g) {
+    uint32_t gp = gen | (prop << 1);
+    lookahead_step<G, 1>(gp, g);  // gp & 1: a carry leaves lanes [0, g]
+    const uint32_t cin = gup<G>(gp & 1u);
+    return g == 0 ? 0u : cin;
+}
+
 // Carry-normalise unnormalised accumulators t (this lane's S positions) into
-// digits; the carry crosses lanes in G - 1 rounds.
+// digits: each lane carries through its own positions, adds the carry out
+// of the lane below (after which at most one bit leaves any lane, S >= 2),
+// and the one-bit carries cross the group by lookahead.
 template <int S, int G>
 __device__ __forceinline__ void normalize(const uint64_t (&t)[S], uint32_t (&x)[S], uint32_t g) {
+    static_assert(S >= 2, "one-bit carries after the first exchange");
     uint64_t c = 0;
 #pragma unroll
     for (int p = 0; p < S; ++p) {
@@ -124,53 +155,51 @@ __device__ __forceinline__ void normalize(const uint64_t (&t)[S], uint32_t (&x)[
         x[p] = (uint32_t)v & MASK;
         c = v >> RB;
     }
+    uint64_t carry = gup64<G>(c);
+    if (g == 0) carry = 0;
+    uint32_t all = MASK;
 #pragma unroll
-    for (int r = 0; r < G - 1; ++r) {
-        uint64_t cin = gup64<G>(c);
-        if (g == 0) cin = 0;
-        c = 0;
-        uint64_t carry = cin;
+    for (int p = 0; p < S; ++p) {
+        const uint64_t v = (uint64_t)x[p] + carry;
+        x[p] = (uint32_t)v & MASK;
+        carry = v >> RB;
+        all &= x[p];
+    }
+    uint32_t cin = carry_in<G>((uint32_t)carry, all == MASK ? 1u : 0u, g);
 #pragma unroll
-        for (int p = 0; p < S; ++p) {
-            const uint64_t v = (uint64_t)x[p] + carry;
-            x[p] = (uint32_t)v & MASK;
-            carry = v >> RB;
-        }
-        c = carry;
+    for (int p = 0; p < S; ++p) {
+        const uint32_t v = x[p] + cin;
+        x[p] = v & MASK;
+        cin = v >> RB;
     }
 }
 
 // y <- y - n over the group; returns the final borrow (1 if y < n), valid in
-// every lane.  Digits of y are left as y - n mod R (caller selects).
+// every lane.  Digits of y are left as y - n mod R (caller selects).  Each
+// lane subtracts its own digits (borrow out 0 or 1); a borrow into a lane
+// passes through it only when its difference digits are all zero; the
+// borrows cross the group by lookahead.
 template <int S, int G>
 __device__ __forceinline__ uint32_t sub_n(const uint32_t (&y)[S], const uint32_t (&n)[S], uint32_t (&u)[S],
                                           uint32_t g) {
-    uint32_t br = 0;
+    uint32_t br = 0, any = 0;
 #pragma unroll
     for (int p = 0; p < S; ++p) {
         const uint32_t v = y[p] - n[p] - br;
         br = v >> 31;
         u[p] = v & MASK;
+        any |= u[p];
     }
-    // Borrows ripple up one lane per round.  A lane that already borrowed
-    // locally holds u >= 1 in its own digits, so an incoming borrow cannot
-    // make it borrow twice: only newly caused borrows travel on.
-    uint32_t total = br, pending = br;
+    const uint32_t prop = any == 0 ? 1u : 0u;
+    uint32_t bin = carry_in<G>(br, prop, g);
+    const uint32_t out = br | (prop & bin);  // borrow out of this lane
 #pragma unroll
-    for (int r = 0; r < G - 1; ++r) {
-        uint32_t bin = gup<G>(pending);
-        if (g == 0) bin = 0;
-        uint32_t b2 = bin;
-#pragma unroll
-        for (int p = 0; p < S; ++p) {
-            const uint32_t v = u[p] - b2;
-            b2 = v >> 31;
-            u[p] = v & MASK;
-        }
-        pending = b2;
-        total |= b2;
+    for (int p = 0; p < S; ++p) {
+        const uint32_t v = u[p] - bin;
+        bin = v >> 31;
+        u[p] = v & MASK;
     }
-    return gbcast_top<G>(total);
+    return gbcast_top<G>(out);
 }
 
 // y <- y mod N for y < 2N.

@@ -135,6 +135,67 @@ struct RegisterLG {
     }
 };
 
+// Strong Lucas with one number per wavefront (lucas_lg.h, WAVE): its four
+// 16-lane rows compute a step's three Montgomery products at once, so a test
+// takes about one product's latency per exponent bit.  LDS: the rows' stages
+// (as for 16 lanes per number), then 64 S exchange words per wavefront.
+template <int S>
+__global__ void __launch_bounds__(256) lucas_wave_kernel(const uint32_t *__restrict__ n, uint32_t limbs,
+                                                         uint32_t count, const uint32_t *__restrict__ d_count,
+                                                         const uint32_t *__restrict__ d_idx,
+                                                         uint8_t *__restrict__ out) {
+    extern __shared__ uint32_t lg_lds[];
+    uint32_t *stage = group_stage<S, 16>(lg_lds);
+    uint32_t *xch = lg_lds + 256 * lg::stage_stride(S) + (threadIdx.x >> 6) * 64 * S;
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t item = t >> 6, lane = t & 63u;
+    const uint32_t cnt = d_count ? min(*d_count, count) : count;
+    if (item >= cnt) return;
+    const uint32_t row = d_idx ? d_idx[item] : item;
+    const uint32_t v =
+        lg::strong_lucas<S, 16, true>(n + (size_t)row * limbs, limbs, lane & 15u, stage, lane >> 4, xch);
+    if (lane == 0) out[row] = (uint8_t)v;
+}
+
+template <int S>
+size_t lucas_wave_lds(int blocks_cap) {
+    const size_t bytes = std::max(lg::stage_bytes(S) + (size_t)256 * S * sizeof(uint32_t),
+                                  occupancy_cap_bytes(lucas_wave_kernel<S>, blocks_cap));
+    if (bytes > 64 * 1024)
+        (void)hipFuncSetAttribute((const void *)lucas_wave_kernel<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)bytes);
+    return bytes;
+}
+
+template <int S>
+hipError_t launch_lucas_wave(const uint32_t *d_n, uint32_t limbs, uint32_t count, const uint32_t *d_count,
+                             const uint32_t *d_idx, Bases, uint8_t *d_out, int blocks_cap, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    const dim3 grid((uint32_t)(((size_t)count * 64 + 255) / 256));
+    hipLaunchKernelGGL(lucas_wave_kernel<S>, grid, dim3(256), lucas_wave_lds<S>(blocks_cap), stream, d_n, limbs,
+                       count, d_count, d_idx, d_out);
+    return hipGetLastError();
+}
+
+template <int S>
+int occupancy_lucas_wave() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lucas_wave_kernel<S>, 256, lucas_wave_lds<S>(0)) !=
+            hipSuccess ||
+        n <= 0)
+        n = 1;
+    return n;
+}
+
+template <int... Ss>
+struct RegisterLucasWave {
+    RegisterLucasWave() {
+        (register_variant(Variant{lg::RB, 16 * Ss, 64, KIND_LUCAS, lg::RB * 16 * Ss - lg::kLucasHeadroom,
+                                  &launch_lucas_wave<Ss>, &occupancy_lucas_wave<Ss>}),
+         ...);
+    }
+};
+
 // Strong Lucas variants (batch launch only; the scan launches them for its
 // Baillie-PSW confirmation rows through the same entry point).
 template <int G, int... Ss>

@@ -64,9 +64,13 @@ constexpr uint32_t MAX_SLOTS = 1u << 16;  // m's in flight (the HBM budget may a
 // shallower window.  1511#/2190 at half of capacity: p50/p99 per m 50/139 ms
 // against 126/206 with one-lane rounds; at 0.8 of capacity 144/316 against
 // 142/297 (above the threshold the one-lane rounds take over again).
-constexpr uint32_t LATENCY_SIDES = 1024;     // scanning sides at most
+constexpr uint32_t LATENCY_SIDES = 2048;     // scanning sides at most
 constexpr uint32_t LATENCY_SIDES_ROW = 256;  // ... and at most this many for 16 lanes instead of 8
 constexpr int LATENCY_MIN_DIGITS = 52;       // one-lane variant at least this wide
+// Strong Lucas batches up to this many tests run one number per wavefront
+// (lowest latency); larger ones on the most lane-efficient variant.
+constexpr uint32_t LUCAS_WAVE_MAX = 256;
+constexpr uint32_t LUCAS_POOLS = 4;  // Lucas batches that may run at once (a ring)
 
 enum : uint32_t {
     F_PREV_DONE = 1,
@@ -75,8 +79,9 @@ enum : uint32_t {
     F_NEXT_WAIT = 8,
     F_SEQ = 16,        // side-skip mode: the next side starts after the prev side's PRP
     F_NEXT_SKIP = 32,  // side-skip taken: the next side was not scanned
-    F_PREV_CONF = 64,  // confirmation rows of the pending hit are in this round
+    F_PREV_CONF = 64,  // the pending hit's confirmations are issued and not all back
     F_NEXT_CONF = 128,
+    F_FINISHED = 256,  // the finished event was written (a second update pass skips the slot)
 };
 
 struct SlotState {
@@ -86,7 +91,8 @@ struct SlotState {
     uint32_t flags;
     uint32_t cwords;
     uint32_t cand[2];   // first Fermat hit awaiting confirmation (NONE if none)
-    uint32_t stage[2];  // confirmation bases it passed in the round testing it
+    uint32_t stage[2];  // confirmations it passed so far
+    uint32_t done[2];   // confirmations whose verdict is back
     uint32_t skip_below;  // F_SEQ: a prev distance below this skips the next side
     uint32_t pad;
 };
@@ -136,6 +142,7 @@ __global__ void scatter_kernel(const Upload *__restrict__ up, const uint32_t *__
             s.found[0] = s.found[1] = NONE;
             s.cand[0] = s.cand[1] = NONE;
             s.stage[0] = s.stage[1] = 0;
+            s.done[0] = s.done[1] = 0;
             s.len[0] = min(d.n0, cap);
             s.len[1] = min(d.n1, cap);
             s.flags = d.init_flags;
@@ -160,115 +167,10 @@ __global__ void scatter_kernel(const Upload *__restrict__ up, const uint32_t *__
     }
 }
 
-// What a side does this round: 0 nothing, 1 the confirmation tests of its
-// pending hit (every base at once, one row each), 2 Fermat tests of its next
-// unknowns.
-__device__ __forceinline__ uint32_t side_role(const SlotState &s, uint32_t side) {
-    const uint32_t done = side ? F_NEXT_DONE : F_PREV_DONE;
-    const uint32_t wait = side ? F_NEXT_WAIT : F_PREV_WAIT;
-    if (s.flags & (done | wait)) return 0;
-    if (side == 1 && (s.flags & (F_SEQ | F_PREV_DONE)) == F_SEQ) return 0;  // next waits for prev
-    if (s.cand[side] != NONE) return 1;
-    return s.cursor[side] < s.len[side] ? 2 : 0;
-}
-
-// Share weight of a scanning side in a round: a side that has issued 128
-// candidates without a confirmed PRP (the tail of the gap distribution) gets
-// twice the share of the others, which cuts the rounds the longest gaps
-// take.  1511#/2190 at half of capacity: p99 134 -> 120 ms, p50 unchanged
-// (49 ms); 3001#/2190: p99 298 -> 261 ms, p50 133 -> 141 ms.
-__device__ __forceinline__ uint32_t side_weight(uint32_t cursor) { return 1u + min(cursor / 128u, 1u); }
-
-// ctr[1] = sides that run Fermat tests this round, ctr[4] = confirmation
-// rows, ctr[5] = the scanning sides' total weight (the Fermat rows divide
-// the resident lanes the confirmations leave among the scanning sides).
-__global__ void count_kernel(const uint32_t *__restrict__ active, uint32_t na, const SlotState *__restrict__ st,
-                             uint32_t nconf, uint32_t *__restrict__ ctr) {
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t side = t & 1u;
-    const SlotState *s = t < 2 * na ? &st[active[t >> 1]] : nullptr;
-    const uint32_t role = s ? side_role(*s, side) : 0u;
-    const uint64_t bs = __ballot(role == 2), bc = __ballot(role == 1);
-    uint32_t w = role == 2 ? side_weight(s->cursor[side]) : 0u;
-    for (int off = 32; off > 0; off >>= 1) w += (uint32_t)__shfl_xor((int)w, off, 64);
-    if ((threadIdx.x & 63u) == 0) {
-        if (bs) atomicAdd(&ctr[1], (uint32_t)__popcll(bs));
-        if (bc) atomicAdd(&ctr[4], (uint32_t)__popcll(bc) * nconf);
-        if (w) atomicAdd(&ctr[5], w);
-    }
-}
-
-// One thread per (active slot, side): for a pending Fermat hit, `conf.n`
-// strong-PRP confirmation items, one per base (rows [0, cr), counter ctr[3]),
-// plus a strong Lucas item when `lucas` is set (rows lr0 + [0, ctr[6])); or
-// Fermat items (rows cr + [0, ctr[0])).  Each scanning side takes its weighted share
-// (side_weight, total ctr[5]) of the resident lanes the confirmations (padded
-// to whole wavefronts of `cpad` items) leave, at least one, capped by `depth`
-// and by the `maxi` rows the round holds: a grid larger than the resident
-// lanes would run a second, mostly empty wave and double the round's time.
-__global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, SlotState *__restrict__ st,
-                             uint32_t cr, uint32_t depth, uint32_t resident, uint32_t cpad, uint32_t maxi, Bases conf,
-                             uint32_t lr0, uint32_t lucas, uint32_t *__restrict__ items,
-                             uint32_t *__restrict__ item_idx, uint8_t *__restrict__ item_base,
-                             uint32_t *__restrict__ ctr) {
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= 2 * na) return;
-    const uint32_t slot = active[t >> 1], side = t & 1u;
-    SlotState &s = st[slot];
-    const uint32_t role = side_role(s, side);
-    if (role == 0) return;
-    if (role == 1) {
-        const uint32_t r = atomicAdd(&ctr[3], conf.n);
-        for (uint32_t j = 0; j < conf.n; ++j) {
-            items[r + j] = (slot << 1) | side;
-            item_idx[r + j] = s.cand[side];
-            item_base[r + j] = (uint8_t)conf.b[j];
-        }
-        if (lucas) {
-            const uint32_t l = lr0 + atomicAdd(&ctr[6], 1u);
-            items[l] = (slot << 1) | side;
-            item_idx[l] = s.cand[side];
-        }
-        s.stage[side] = 0;
-        atomicOr(&s.flags, side ? F_NEXT_CONF : F_PREV_CONF);  // the other side's thread writes flags too
-        return;
-    }
-    const uint32_t c = s.cursor[side], l = s.len[side];
-    const uint32_t ns = max(ctr[1], 1u), wsum = max(ctr[5], 1u), w = side_weight(c);
-    const uint32_t used = (ctr[4] + cpad - 1) / cpad * cpad;
-    const uint32_t room = resident > used ? resident - used : 0u;
-    // weighted shares, at least one item each: the sum stays within room
-    // (when room >= ns) and within maxi (maxi >= 2 na >= ns)
-    const uint32_t share_room = room > ns ? (uint32_t)((uint64_t)(room - ns) * w / wsum) + 1u : 1u;
-    const uint32_t share_max = (uint32_t)((uint64_t)(maxi - ns) * w / wsum) + 1u;
-    const uint32_t d = min(min(depth, share_room), share_max);
-    const uint32_t k = min(d, l - c);
-    const uint32_t base = cr + atomicAdd(&ctr[0], k);
-    for (uint32_t j = 0; j < k; ++j) {
-        items[base + j] = (slot << 1) | side;
-        item_idx[base + j] = c + j;
-    }
-    s.cursor[side] = c + k;
-}
-
-// Rows in use this round: strong-PRP confirmations [0, ctr[3]), Fermat
-// cr + [0, ctr[0]), strong Lucas confirmations lr0 + [0, ctr[6]).
-__device__ __forceinline__ bool row_live(uint32_t t, uint32_t cr, uint32_t lr0, const uint32_t *ctr) {
-    return t < cr ? t < ctr[3] : (t < lr0 ? t - cr < ctr[0] : t - lr0 < ctr[6]);
-}
-
-// N = centre -+ x for every item, `w` words per row.
-__global__ void materialize_kernel(const uint32_t *__restrict__ items, const uint32_t *__restrict__ item_idx,
-                                   uint32_t cr, uint32_t lr0, uint32_t nrows, const uint32_t *__restrict__ ctr,
-                                   const uint32_t *__restrict__ center, uint32_t cw,
-                                   const int32_t *__restrict__ lists, uint32_t cap, uint32_t *__restrict__ nbuf,
-                                   uint32_t w) {
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= nrows || !row_live(t, cr, lr0, ctr)) return;
-    const uint32_t slot = items[t] >> 1, side = items[t] & 1u;
-    const uint32_t x = (uint32_t)lists[((size_t)slot * 2 + side) * cap + item_idx[t]];
-    const uint32_t *c = center + (size_t)slot * cw;
-    uint32_t *o = nbuf + (size_t)t * w;
+// N = centre + x (side 1) or centre - x (side 0) as `w` words at o; the
+// centre has cw words (zero above).
+__device__ __forceinline__ void write_n(const uint32_t *__restrict__ c, uint32_t cw, uint32_t side, uint32_t x,
+                                       uint32_t *__restrict__ o, uint32_t w) {
     if (side) {
         uint64_t carry = x;
         for (uint32_t j = 0; j < w; ++j) {
@@ -286,23 +188,164 @@ __global__ void materialize_kernel(const uint32_t *__restrict__ items, const uin
     }
 }
 
-// Confirmation rows count the tests their side's pending hit passed (the
-// update decides); Fermat hits become the side's pending hit (or, with no
-// confirmation, its PRP).  A side has either confirmation rows or Fermat
-// rows in a round, never both.
+// What a side does this round: 0 nothing, 1 the confirmation tests of its
+// pending hit (every base at once, one row each), 2 Fermat tests of its next
+// unknowns.
+__device__ __forceinline__ uint32_t side_role(const SlotState &s, uint32_t side) {
+    const uint32_t done = side ? F_NEXT_DONE : F_PREV_DONE;
+    const uint32_t wait = side ? F_NEXT_WAIT : F_PREV_WAIT;
+    if (s.flags & (done | wait)) return 0;
+    if (side == 1 && (s.flags & (F_SEQ | F_PREV_DONE)) == F_SEQ) return 0;  // next waits for prev
+    if (s.flags & (side ? F_NEXT_CONF : F_PREV_CONF)) return 0;              // verdicts outstanding
+    if (s.cand[side] != NONE) return 1;
+    return s.cursor[side] < s.len[side] ? 2 : 0;
+}
+
+// Share weight of a scanning side in a round: a side that has issued 128
+// candidates without a confirmed PRP (the tail of the gap distribution) gets
+// twice the share of the others, which cuts the rounds the longest gaps
+// take.  1511#/2190 at half of capacity: p99 134 -> 120 ms, p50 unchanged
+// (49 ms); 3001#/2190: p99 298 -> 261 ms, p50 133 -> 141 ms.
+__device__ __forceinline__ uint32_t side_weight(uint32_t cursor) { return 1u + min(cursor / 128u, 1u); }
+
+// ctr[1] = sides that run Fermat tests this round, ctr[4] = confirmation
+// rows, ctr[5] = the scanning sides' total weight, ctr[6] = confirming sides
+// (with Baillie-PSW, each starts a strong Lucas test that takes a whole
+// wavefront beside the grid; the Fermat rows divide the resident lanes the
+// confirmations leave among the scanning sides).
+__global__ void count_kernel(const uint32_t *__restrict__ active, uint32_t na, const SlotState *__restrict__ st,
+                             uint32_t nconf, uint32_t *__restrict__ ctr) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t side = t & 1u;
+    const SlotState *s = t < 2 * na ? &st[active[t >> 1]] : nullptr;
+    const uint32_t role = s ? side_role(*s, side) : 0u;
+    const uint64_t bs = __ballot(role == 2), bc = __ballot(role == 1);
+    uint32_t w = role == 2 ? side_weight(s->cursor[side]) : 0u;
+    for (int off = 32; off > 0; off >>= 1) w += (uint32_t)__shfl_xor((int)w, off, 64);
+    if ((threadIdx.x & 63u) == 0) {
+        if (bs) atomicAdd(&ctr[1], (uint32_t)__popcll(bs));
+        if (bc) {
+            atomicAdd(&ctr[4], (uint32_t)__popcll(bc) * nconf);
+            atomicAdd(&ctr[6], (uint32_t)__popcll(bc));
+        }
+        if (w) atomicAdd(&ctr[5], w);
+    }
+}
+
+// One thread per (active slot, side): for a pending Fermat hit, `conf.n`
+// strong-PRP confirmation items, one per base (rows [0, cr), counter ctr[3]),
+// plus, when `lpool` is set, a strong Lucas item appended to the Lucas pool
+// (item, and its number N = centre -+ x written here: `lw` words at
+// lnum + index * lw, count *lcount); or Fermat items (rows cr + [0, ctr[0])).  Each scanning side takes its weighted share
+// (side_weight, total ctr[5]) of the resident lanes the confirmations (padded
+// to whole wavefronts of `cpad` items) leave, at least one, capped by `depth`
+// and by the `maxi` rows the round holds: a grid larger than the resident
+// lanes would run a second, mostly empty wave and double the round's time.
+__global__ void issue_kernel(const uint32_t *__restrict__ active, uint32_t na, SlotState *__restrict__ st,
+                             uint32_t cr, uint32_t depth, uint32_t resident, uint32_t cpad, uint32_t maxi, Bases conf,
+                             uint32_t *__restrict__ lpool, uint32_t *__restrict__ lcount, uint32_t lres,
+                             uint32_t *__restrict__ lnum, uint32_t lw, const uint32_t *__restrict__ center,
+                             uint32_t cw, const int32_t *__restrict__ lists, uint32_t cap,
+                             uint32_t *__restrict__ items, uint32_t *__restrict__ item_idx,
+                             uint8_t *__restrict__ item_base, uint32_t *__restrict__ ctr) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= 2 * na) return;
+    const uint32_t slot = active[t >> 1], side = t & 1u;
+    SlotState &s = st[slot];
+    const uint32_t role = side_role(s, side);
+    if (role == 0) return;
+    if (role == 1) {
+        const uint32_t r = atomicAdd(&ctr[3], conf.n);
+        for (uint32_t j = 0; j < conf.n; ++j) {
+            items[r + j] = (slot << 1) | side;
+            item_idx[r + j] = s.cand[side];
+            item_base[r + j] = (uint8_t)conf.b[j];
+        }
+        if (lpool) {
+            const uint32_t l = atomicAdd(lcount, 1u);
+            lpool[l] = (slot << 1) | side;
+            const uint32_t x = (uint32_t)lists[((size_t)slot * 2 + side) * cap + s.cand[side]];
+            write_n(center + (size_t)slot * cw, cw, side, x, lnum + (size_t)l * lw, lw);
+        }
+        s.stage[side] = 0;
+        s.done[side] = 0;
+        atomicOr(&s.flags, side ? F_NEXT_CONF : F_PREV_CONF);  // the other side's thread writes flags too
+        return;
+    }
+    const uint32_t c = s.cursor[side], l = s.len[side];
+    const uint32_t ns = max(ctr[1], 1u), wsum = max(ctr[5], 1u), w = side_weight(c);
+    // confirmation rows in whole wavefronts, and the lanes of the strong Lucas
+    // tests that run beside the grid (`lres` lanes each, in units of this
+    // grid's items of 64 / cpad lanes)
+    const uint32_t used = (ctr[4] + cpad - 1) / cpad * cpad + (lpool ? (ctr[6] * lres * cpad + 63) / 64 : 0u);
+    const uint32_t room = resident > used ? resident - used : 0u;
+    // weighted shares, at least one item each: the sum stays within room
+    // (when room >= ns) and within maxi (maxi >= 2 na >= ns)
+    const uint32_t share_room = room > ns ? (uint32_t)((uint64_t)(room - ns) * w / wsum) + 1u : 1u;
+    const uint32_t share_max = (uint32_t)((uint64_t)(maxi - ns) * w / wsum) + 1u;
+    const uint32_t d = min(min(depth, share_room), share_max);
+    const uint32_t k = min(d, l - c);
+    const uint32_t base = cr + atomicAdd(&ctr[0], k);
+    for (uint32_t j = 0; j < k; ++j) {
+        items[base + j] = (slot << 1) | side;
+        item_idx[base + j] = c + j;
+    }
+    s.cursor[side] = c + k;
+}
+
+// Rows in use this round: strong-PRP confirmations [0, ctr[3]), Fermat
+// cr + [0, ctr[0]).
+__device__ __forceinline__ bool row_live(uint32_t t, uint32_t cr, const uint32_t *ctr) {
+    return t < cr ? t < ctr[3] : t - cr < ctr[0];
+}
+
+// N = centre -+ x for every item, `w` words per row.
+__global__ void materialize_kernel(const uint32_t *__restrict__ items, const uint32_t *__restrict__ item_idx,
+                                   uint32_t cr, uint32_t nrows, const uint32_t *__restrict__ ctr,
+                                   const uint32_t *__restrict__ center, uint32_t cw,
+                                   const int32_t *__restrict__ lists, uint32_t cap, uint32_t *__restrict__ nbuf,
+                                   uint32_t w) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= nrows || !row_live(t, cr, ctr)) return;
+    const uint32_t slot = items[t] >> 1, side = items[t] & 1u;
+    const uint32_t x = (uint32_t)lists[((size_t)slot * 2 + side) * cap + item_idx[t]];
+    write_n(center + (size_t)slot * cw, cw, side, x, nbuf + (size_t)t * w, w);
+}
+
+// Confirmation rows count the tests their side's pending hit passed and the
+// verdicts back (the update decides); Fermat hits become the side's pending
+// hit (or, with no confirmation, its PRP).  A side has either confirmation
+// rows or Fermat rows in a round, never both.
 __global__ void retire_kernel(const uint32_t *__restrict__ items, const uint32_t *__restrict__ item_idx,
-                              uint32_t cr, uint32_t lr0, uint32_t nrows, const uint8_t *__restrict__ verdict,
+                              uint32_t cr, uint32_t nrows, const uint8_t *__restrict__ verdict,
                               const uint32_t *__restrict__ ctr, uint32_t confirm, SlotState *__restrict__ st) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= nrows || !row_live(t, cr, lr0, ctr)) return;
+    if (t >= nrows || !row_live(t, cr, ctr)) return;
     const uint32_t slot = items[t] >> 1, side = items[t] & 1u;
     SlotState &s = st[slot];
-    if (t < cr || t >= lr0) {
+    if (t < cr) {
         if (verdict[t]) atomicAdd(&s.stage[side], 1u);
+        atomicAdd(&s.done[side], 1u);
         return;
     }
     if (!verdict[t]) return;
     atomicMin(confirm ? &s.cand[side] : &s.found[side], item_idx[t]);
+}
+
+// A finished strong Lucas batch: each item's verdict counts toward its
+// side's pending confirmations.
+__global__ void lucas_retire_kernel(const uint32_t *__restrict__ pool, const uint8_t *__restrict__ verdict,
+                                    uint32_t count, SlotState *__restrict__ st) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= count) return;
+    const uint32_t slot = pool[t] >> 1, side = pool[t] & 1u;
+    if (verdict[t]) atomicAdd(&st[slot].stage[side], 1u);
+    atomicAdd(&st[slot].done[side], 1u);
+}
+
+// Counter reset as a kernel (the round sequence carries no memset nodes).
+__global__ void zero_word_kernel(uint32_t *__restrict__ p) {
+    if (threadIdx.x == 0) *p = 0;
 }
 
 __global__ void update_kernel(const uint32_t *__restrict__ active, uint32_t na, SlotState *__restrict__ st,
@@ -313,11 +356,12 @@ __global__ void update_kernel(const uint32_t *__restrict__ active, uint32_t na, 
     const uint32_t slot = active[t];
     SlotState &s = st[slot];
     uint32_t flags = s.flags, code = 0;
-    // a confirmation tested this round: every test passed -> the side's PRP;
-    // any failed (a base-2 pseudoprime) -> resume just past it
+    if (flags & F_FINISHED) return;
+    // every confirmation verdict back: all passed -> the side's PRP; any
+    // failed (a base-2 pseudoprime) -> resume just past it
     for (uint32_t side = 0; side < 2; ++side) {
         const uint32_t conf = side ? F_NEXT_CONF : F_PREV_CONF;
-        if (!(flags & conf)) continue;
+        if (!(flags & conf) || s.done[side] < nconf) continue;
         flags &= ~conf;
         if (s.stage[side] >= nconf) {
             s.found[side] = s.cand[side];
@@ -326,6 +370,7 @@ __global__ void update_kernel(const uint32_t *__restrict__ active, uint32_t na, 
             s.cand[side] = NONE;
         }
         s.stage[side] = 0;
+        s.done[side] = 0;
     }
     for (uint32_t side = 0; side < 2; ++side) {
         const uint32_t done = side ? F_NEXT_DONE : F_PREV_DONE;
@@ -343,8 +388,11 @@ __global__ void update_kernel(const uint32_t *__restrict__ active, uint32_t na, 
             code |= 2u << side;
         }
     }
+    if ((flags & (F_PREV_DONE | F_NEXT_DONE)) == (F_PREV_DONE | F_NEXT_DONE)) {
+        code |= 1u;
+        flags |= F_FINISHED;
+    }
     s.flags = flags;
-    if ((flags & (F_PREV_DONE | F_NEXT_DONE)) == (F_PREV_DONE | F_NEXT_DONE)) code |= 1u;
     if (!code) return;
     Event ev;
     ev.slot = slot;
@@ -439,10 +487,25 @@ class Scanner {
     bool stop = false;
     std::thread worker;
     hipStream_t stream = nullptr;      // the worker's stream
-    // the strong Lucas rows run beside the round's grid on a stream of their
-    // own (default priority), joined before the retire step
+    // Strong Lucas confirmations (Baillie-PSW) are not part of a round's
+    // grid: a Lucas test costs more than a Fermat test, and a round is sized
+    // to one.  A round's Lucas tests (one per confirming side) go to a pool
+    // of a ring of LUCAS_POOLS (the item and its number), which starts as a
+    // batch right after the issue step on a stream of its own (default
+    // priority; batches run there in order); finished batches are retired,
+    // oldest first, at the end of a round, before a second update pass, so
+    // the m's they decide finish in that round.  A side waits for all its
+    // confirmations while the other sides keep scanning.
     hipStream_t lstream = nullptr;
-    hipEvent_t ev_rows = nullptr, ev_lucas = nullptr;
+    hipEvent_t ev_rows = nullptr, ev_lucas[LUCAS_POOLS] = {};
+    uint32_t *d_lpool = nullptr, *d_lnum = nullptr, *d_lctr = nullptr;
+    uint8_t *d_lverd = nullptr;
+    uint32_t LP = 0;                       // items per pool (two per slot)
+    uint32_t lhead = 0, lbusy = 0;         // oldest running pool, pools running
+    uint32_t lcount[LUCAS_POOLS] = {};
+    // latency-first rounds are chosen by the sides that scan: the last
+    // round's count plus two per m admitted since
+    uint32_t last_scanning = 0, admitted_last = 0;
 
     std::vector<SlotHost> slots;
     std::vector<uint32_t> free_slots, active;
@@ -503,11 +566,21 @@ class Scanner {
         h_events = nullptr;
         h_up = nullptr;
         if (lstream) (void)hipStreamSynchronize(lstream);
+        for (void *p : {(void *)d_lpool, (void *)d_lnum, (void *)d_lctr, (void *)d_lverd})
+            if (p) (void)hipFree(p);
+        d_lpool = d_lnum = d_lctr = nullptr;
+        d_lverd = nullptr;
+        lhead = lbusy = 0;
+        last_scanning = admitted_last = 0;
+        for (uint32_t p = 0; p < LUCAS_POOLS; ++p) {
+            lcount[p] = 0;
+            if (ev_lucas[p]) (void)hipEventDestroy(ev_lucas[p]);
+            ev_lucas[p] = nullptr;
+        }
         if (ev_rows) (void)hipEventDestroy(ev_rows);
-        if (ev_lucas) (void)hipEventDestroy(ev_lucas);
         if (lstream) (void)hipStreamDestroy(lstream);
         if (stream) (void)hipStreamDestroy(stream);
-        ev_rows = ev_lucas = nullptr;
+        ev_rows = nullptr;
         lstream = nullptr;
         stream = nullptr;
         inited = false;
@@ -592,11 +665,11 @@ class Scanner {
         }
         const size_t row_bytes = 4ull * W + 4 + 4 + 1;  // number, item, index, verdict
         // per slot: state, centre, lists, active entry, event, and per side
-        // one strong-PRP confirmation row (+ its base), one strong Lucas row
-        // and one Fermat row
+        // one strong-PRP confirmation row (+ its base), one Fermat row and
+        // an item in each strong Lucas pool
         const size_t nconf = 1, nlucas = confirm == PGAP_CONFIRM_BPSW ? 1 : 0;
         const size_t per_slot = sizeof(SlotState) + 4ull * CW + 8ull * CAP + 4 + sizeof(Event) +
-                                2ull * (nconf * (row_bytes + 1) + nlucas * row_bytes + row_bytes);
+                                2ull * (nconf * (row_bytes + 1) + row_bytes) + nlucas * 2ull * LUCAS_POOLS * row_bytes;
         size_t budget = e->opt.hbm_budget;
         if (budget == 0) {
             size_t fr = 0, tot = 0;
@@ -611,8 +684,9 @@ class Scanner {
             return PGAP_E_TOO_LARGE;
         }
         MAXI = (uint32_t)std::min<size_t>(extra + 2ull * S, 0x7fffffffu);
-        // strong-PRP confirmation rows, Fermat rows, strong Lucas rows
-        const size_t rows = 2ull * S * nconf + MAXI + 2ull * S * nlucas;
+        // strong-PRP confirmation rows, then Fermat rows
+        const size_t rows = 2ull * S * nconf + MAXI;
+        LP = 2 * S;
         pgap_status st;
         if ((st = dalloc(&d_state, S, "state")) || (st = dalloc(&d_center, (size_t)S * CW, "centres")) ||
             (st = dalloc(&d_lists, (size_t)S * 2 * CAP, "lists")) || (st = dalloc(&d_active, S, "active")) ||
@@ -620,7 +694,11 @@ class Scanner {
             (st = dalloc(&d_nbuf, rows * W, "numbers")) || (st = dalloc(&d_verdict, rows, "verdicts")) ||
             (st = dalloc(&d_base, 2ull * S * nconf, "confirmation bases")) || (st = dalloc(&d_ctr, 8, "counters")) ||
             (st = dalloc(&d_events, S, "events")) || (st = halloc(&h_active, S, "active")) ||
-            (st = halloc(&h_ctr, 8, "counters")) || (st = halloc(&h_events, S, "events"))) {
+            (st = halloc(&h_ctr, 16, "counters")) || (st = halloc(&h_events, S, "events")) ||
+            (nlucas && ((st = dalloc(&d_lpool, (size_t)LUCAS_POOLS * LP, "Lucas pools")) ||
+                        (st = dalloc(&d_lnum, (size_t)LUCAS_POOLS * LP * W, "Lucas pool numbers")) ||
+                        (st = dalloc(&d_lverd, (size_t)LUCAS_POOLS * LP, "Lucas pool verdicts")) ||
+                        (st = dalloc(&d_lctr, 64, "Lucas pool counters"))))) {
             release();
             return st;
         }
@@ -628,9 +706,15 @@ class Scanner {
         if ((he = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess ||
             (he = hipStreamCreateWithFlags(&lstream, hipStreamNonBlocking)) != hipSuccess ||
             (he = hipEventCreateWithFlags(&ev_rows, hipEventDisableTiming)) != hipSuccess ||
-            (he = hipEventCreateWithFlags(&ev_lucas, hipEventDisableTiming)) != hipSuccess) {
+            (d_lctr && (he = hipMemset(d_lctr, 0, 64 * sizeof(uint32_t))) != hipSuccess)) {
             release();
             return e->hip_fail(he, "scan streams");
+        }
+        for (uint32_t p = 0; p < LUCAS_POOLS; ++p) {
+            if ((he = hipEventCreateWithFlags(&ev_lucas[p], hipEventDisableTiming)) != hipSuccess) {
+                release();
+                return e->hip_fail(he, "scan streams");
+            }
         }
         slots.assign(S, SlotHost());
         free_slots.clear();
@@ -738,6 +822,7 @@ class Scanner {
             }
             admitted += take.size();  // no window where an m is neither staged nor admitted
         }
+        admitted_last = (uint32_t)take.size();
         const size_t nup = take.size() + pending_append.size();
         if (nup == 0) return PGAP_OK;
         size_t words = 0;
@@ -809,6 +894,76 @@ class Scanner {
         return PGAP_OK;
     }
 
+    // The variant for a Lucas batch of about n tests of up to `bits` bits: a
+    // small batch takes one number per wavefront (a step's three products at
+    // once: the shortest latency, and the m's waiting on it finish sooner);
+    // a large one the most lane-efficient variant (it shares the device with
+    // the rounds' grids).
+    pgap_status lucas_choice(uint32_t n, uint32_t bits, Choice *c) {
+        const Variant *vw = e->opt.lanes_per_number || e->opt.radix || n > LUCAS_WAVE_MAX
+                                ? nullptr
+                                : find_variant(KIND_LUCAS, (int)bits, 28, 64);
+        if (vw) {
+            *c = Choice{vw, 0};
+            return PGAP_OK;
+        }
+        return e->pick(KIND_LUCAS, bits, c);
+    }
+
+    // Start pool p (n items, written by the issue step already queued on the
+    // round's stream) as a batch on the Lucas stream.
+    pgap_status lucas_launch(uint32_t p, const Choice &c, uint32_t n) {
+        Bases none{};
+        hipError_t he;
+        if ((he = hipEventRecord(ev_rows, stream)) != hipSuccess ||
+            (he = hipStreamWaitEvent(lstream, ev_rows, 0)) != hipSuccess ||
+            (he = c.v->launch(d_lnum + (size_t)p * LP * W, W, n, nullptr, nullptr, none, d_lverd + (size_t)p * LP,
+                              c.cap, lstream)) != hipSuccess ||
+            (he = hipEventRecord(ev_lucas[p], lstream)) != hipSuccess)
+            return e->hip_fail(he, "strong Lucas batch launch");
+        lcount[p] = n;
+        ++lbusy;
+        return PGAP_OK;
+    }
+
+    // Retire the finished Lucas batches, oldest first (`wait`: wait for the
+    // oldest one): their verdicts count toward their sides' confirmations.
+    // With `update`, queue a second update pass on the round's stream when
+    // any was retired (at the end of a round; at its start the round's own
+    // update decides).  `any`: whether one was.
+    pgap_status lucas_retire(uint32_t na, uint32_t nconf, bool wait, bool update, bool *any, uint64_t *issued) {
+        *any = false;
+        while (lbusy) {
+            const uint32_t p = lhead;
+            hipError_t he;
+            if (wait && !*any) {
+                if ((he = hipEventSynchronize(ev_lucas[p])) != hipSuccess)
+                    return e->hip_fail(he, "strong Lucas batch");
+            } else if ((he = hipEventQuery(ev_lucas[p])) == hipErrorNotReady) {
+                break;
+            } else if (he != hipSuccess) {
+                return e->hip_fail(he, "strong Lucas batch");
+            }
+            const uint32_t n = lcount[p];
+            const dim3 b256(256);
+            hipLaunchKernelGGL(lucas_retire_kernel, dim3((n + 255) / 256), b256, 0, stream,
+                               d_lpool + (size_t)p * LP, d_lverd + (size_t)p * LP, n, d_state);
+            hipLaunchKernelGGL(zero_word_kernel, dim3(1), dim3(64), 0, stream, d_lctr + p);
+            *issued += n;
+            lcount[p] = 0;
+            lhead = (lhead + 1) % LUCAS_POOLS;
+            --lbusy;
+            *any = true;
+        }
+        if (*any && update) {
+            hipLaunchKernelGGL(update_kernel, dim3((na + 255) / 256), dim3(256), 0, stream, d_active, na, d_state,
+                               d_lists, CAP, nconf, d_ctr, d_events);
+            hipError_t he = hipGetLastError();
+            if (he != hipSuccess) return e->hip_fail(he, "strong Lucas retire");
+        }
+        return PGAP_OK;
+    }
+
     pgap_status round() {
         pgap_status st;
         if ((st = admit())) return st;
@@ -831,14 +986,15 @@ class Scanner {
             conf.b[0] = 2;
         }
         const bool lucas = confirm == PGAP_CONFIRM_BPSW;
-        Choice cf, cc, cl;
+        Choice cf, cc;
         if ((st = e->pick(KIND_FERMAT2, bits, &cf))) return st;
         if (conf.n && (st = e->pick(KIND_SPRP, bits, &cc))) return st;
-        if (lucas && (st = e->pick(KIND_LUCAS, bits, &cl))) return st;
-        if (2 * na <= LATENCY_SIDES && !e->opt.lanes_per_number && !e->opt.radix) {
+        // sides that will scan this round (m's waiting for confirmations do not)
+        const uint32_t sides = (uint32_t)std::min<uint64_t>(2ull * na, (uint64_t)last_scanning + 2ull * admitted_last);
+        if (sides <= LATENCY_SIDES && !e->opt.lanes_per_number && !e->opt.radix) {
             // wide one-lane numbers move to 8 lanes (16 while very few sides
-            // scan), 8-lane numbers to 16; strong Lucas rows to 16 lanes
-            const bool row = 2 * na <= LATENCY_SIDES_ROW;
+            // scan), 8-lane numbers to 16
+            const bool row = sides <= LATENCY_SIDES_ROW;
             const int lanes = cf.v->lanes == 1 && cf.v->digits >= LATENCY_MIN_DIGITS ? (row ? 16 : 8)
                               : cf.v->lanes == 8                                       ? 16
                                                                                        : 0;
@@ -848,8 +1004,6 @@ class Scanner {
                 cf = Choice{lf, 0};
                 cc = Choice{lc, 0};
             }
-            const Variant *ll = lucas ? find_variant(KIND_LUCAS, (int)bits, 28, 16) : nullptr;
-            if (ll) cl = Choice{ll, 0};
         }
         const Variant *vf = cf.v, *vc = cc.v;
         if (vc && !vc->scan) {
@@ -872,8 +1026,19 @@ class Scanner {
         int (*occ)() = vc ? vc->scan_blocks_per_cu : vf->blocks_per_cu;
         const uint32_t per_wave = e->num_cus * 256u / (uint32_t)std::max(1, vr->lanes);
         const uint32_t resident_full = (uint32_t)std::max(1, occ ? occ() : 1) * per_wave;
-        const uint32_t resident = 2ull * na >= per_wave / 4 ? resident_full : std::min(resident_full, per_wave);
+        const uint32_t resident = 4ull * sides >= per_wave ? resident_full : std::min(resident_full, per_wave);
         const uint32_t cpad = 64u / (uint32_t)std::max(1, vr->lanes);  // items per wavefront
+
+        Choice lc;
+        uint32_t lres = 0, lpool_now = 0, nluc = 0;
+        uint64_t issued = 0;
+        if (lucas && lbusy == LUCAS_POOLS) {
+            // every pool still running: wait for the oldest (the round's own
+            // update decides the m's it confirms)
+            bool any = false;
+            if ((st = lucas_retire(na, conf.n + 1, true, false, &any, &issued))) return st;
+        }
+        if (lucas) lpool_now = (lhead + lbusy) % LUCAS_POOLS;
 
         memcpy(h_active, active.data(), na * 4);
         hipError_t he;
@@ -884,48 +1049,66 @@ class Scanner {
         // Fermat rows: the per-side depth (set on the device from the number
         // of scanning sides) gives at most resident + one row per side
         const uint32_t maxi = (uint32_t)std::min<uint64_t>(MAXI, (uint64_t)resident + 2ull * na);
-        const uint32_t lr0 = cr + maxi;              // strong Lucas rows: one per side at most
-        const uint32_t nl = lucas ? 2 * na : 0;
-        const uint32_t nrows = lr0 + nl;
+        const uint32_t nrows = cr + maxi;
         const dim3 b256(256);
         hipLaunchKernelGGL(count_kernel, dim3((2 * na + 255) / 256), b256, 0, stream, d_active, na, d_state,
                            conf.n, d_ctr);
+        if (lucas) {
+            // this round's strong Lucas batch: one test per confirming side
+            // (just counted); its variant by that number, and, in rounds held
+            // to one wave per SIMD, the lanes it takes beside the grid,
+            // reserved from the Fermat rows
+            if ((he = hipMemcpyAsync(h_ctr, d_ctr, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream)) !=
+                    hipSuccess ||
+                (he = hipStreamSynchronize(stream)) != hipSuccess)
+                return e->hip_fail(he, "scan count");
+            nluc = h_ctr[6];
+            if (nluc) {
+                if ((st = lucas_choice(nluc, bits, &lc))) return st;
+                lres = resident < resident_full ? (uint32_t)lc.v->lanes : 0u;
+            }
+        }
         hipLaunchKernelGGL(issue_kernel, dim3((2 * na + 255) / 256), b256, 0, stream, d_active, na, d_state, cr,
-                           depth_max(), resident, cpad, maxi, conf, lr0, lucas ? 1u : 0u, d_items, d_idx, d_base,
-                           d_ctr);
-        hipLaunchKernelGGL(materialize_kernel, dim3((nrows + 255) / 256), b256, 0, stream, d_items, d_idx, cr, lr0,
+                           depth_max(), resident, cpad, maxi, conf,
+                           lucas ? d_lpool + (size_t)lpool_now * LP : nullptr, lucas ? d_lctr + lpool_now : nullptr,
+                           lres, lucas ? d_lnum + (size_t)lpool_now * LP * W : nullptr, W,
+                           d_center, CW, d_lists, CAP, d_items, d_idx, d_base, d_ctr);
+        hipLaunchKernelGGL(materialize_kernel, dim3((nrows + 255) / 256), b256, 0, stream, d_items, d_idx, cr,
                            nrows, d_ctr, d_center, CW, d_lists, CAP, d_nbuf, W);
         if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "scan issue");
-        if (lucas) {
-            // the round's strong Lucas rows (count on the device in ctr[6]),
-            // concurrent with the grid below: they need only the materialised rows
-            if ((he = hipEventRecord(ev_rows, stream)) != hipSuccess ||
-                (he = hipStreamWaitEvent(lstream, ev_rows, 0)) != hipSuccess)
-                return e->hip_fail(he, "scan Lucas stream wait");
-            he = cl.v->launch(d_nbuf + (size_t)lr0 * W, W, nl, d_ctr + 6, nullptr, none, d_verdict + lr0, cl.cap,
-                              lstream);
-            if (he != hipSuccess) return e->hip_fail(he, "scan strong Lucas launch");
-            if ((he = hipEventRecord(ev_lucas, lstream)) != hipSuccess)
-                return e->hip_fail(he, "scan Lucas event");
-        }
+        // this round's confirmations start their strong Lucas tests now,
+        // beside the grid
+        if (nluc && (st = lucas_launch(lpool_now, lc, nluc))) return st;
         if (vc)
             he = vc->scan(d_nbuf, W, cr, cr, maxi, d_ctr, d_base, d_verdict, cap_c, stream);
         else
             he = vf->launch(d_nbuf + (size_t)cr * W, W, maxi, d_ctr, nullptr, none, d_verdict + cr, cap_f,
                             stream);
         if (he != hipSuccess) return e->hip_fail(he, "scan test launch");
-        if (lucas && (he = hipStreamWaitEvent(stream, ev_lucas, 0)) != hipSuccess)
-            return e->hip_fail(he, "scan Lucas join");
-        hipLaunchKernelGGL(retire_kernel, dim3((nrows + 255) / 256), b256, 0, stream, d_items, d_idx, cr, lr0,
-                           nrows, d_verdict, d_ctr, confirm, d_state);
+        hipLaunchKernelGGL(retire_kernel, dim3((nrows + 255) / 256), b256, 0, stream, d_items, d_idx, cr, nrows,
+                           d_verdict, d_ctr, confirm, d_state);
         hipLaunchKernelGGL(update_kernel, dim3((na + 255) / 256), b256, 0, stream, d_active, na, d_state, d_lists,
                            CAP, conf.n + (lucas ? 1u : 0u), d_ctr, d_events);
         if ((he = hipGetLastError()) != hipSuccess) return e->hip_fail(he, "scan retire");
         if ((he = hipMemcpyAsync(h_ctr, d_ctr, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream)) != hipSuccess ||
             (he = hipStreamSynchronize(stream)) != hipSuccess)
             return e->hip_fail(he, "scan round");
-        const uint32_t nev = h_ctr[2];
-        const uint64_t issued = (uint64_t)h_ctr[0] + h_ctr[3] + h_ctr[6];
+        uint32_t nev = h_ctr[2];
+        issued += (uint64_t)h_ctr[0] + h_ctr[3];
+        last_scanning = h_ctr[1];
+        if (lucas) {
+            // finished Lucas batches decide their m's in this round (when the
+            // round tested nothing, wait for the oldest instead of spinning)
+            bool any = false;
+            if ((st = lucas_retire(na, conf.n + 1, h_ctr[0] + h_ctr[3] == 0, true, &any, &issued))) return st;
+            if (any) {
+                if ((he = hipMemcpyAsync(h_ctr, d_ctr, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream)) !=
+                        hipSuccess ||
+                    (he = hipStreamSynchronize(stream)) != hipSuccess)
+                    return e->hip_fail(he, "strong Lucas retire");
+                nev = h_ctr[2];
+            }
+        }
         if (nev) {
             if ((he = hipMemcpy(h_events, d_events, nev * sizeof(Event), hipMemcpyDeviceToHost)) != hipSuccess)
                 return e->hip_fail(he, "scan events");

@@ -83,7 +83,7 @@ def test_random_sizes(engine, bits):
     assert [int(g) for g in got_b] == [1 if ol.probab_prime(v) else 0 for v in vals]
 
 
-@pytest.mark.parametrize("lanes", [8, 16])
+@pytest.mark.parametrize("lanes", [8, 16, 64])
 def test_lane_families_agree(lanes):
     rng = random.Random(lanes)
     vals = []
