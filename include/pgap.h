@@ -37,7 +37,8 @@ typedef enum {
     PGAP_E_DEVICE = -2,     /* HIP runtime error, or the HIP code object is missing */
     PGAP_E_OOM = -3,        /* device allocation failed / admission refused by the HBM budget */
     PGAP_E_TOO_LARGE = -4,  /* number wider than the largest kernel variant */
-    PGAP_E_STATE = -5       /* call not valid in the engine's current state */
+    PGAP_E_STATE = -5,      /* call not valid in the engine's current state */
+    PGAP_E_TIMEOUT = -6     /* pgap_scan_poll: no row within the time allowed */
 } pgap_status;
 
 /* The test a batch runs.
@@ -205,10 +206,13 @@ typedef struct {
  * join the next round.  Submit and poll may be called from one client thread
  * (not concurrently from several). */
 pgap_status pgap_scan_submit(pgap_engine *e, const pgap_scan_req *req);
-/* Collect finished rows (up to cap).  wait != 0 blocks until at least one
- * row is ready or nothing is in flight.  An error inside the worker is
- * returned by the next poll or submit. */
-pgap_status pgap_scan_poll(pgap_engine *e, pgap_scan_row *rows, size_t cap, size_t *n_out, int wait);
+/* Collect finished rows (up to cap).  timeout_ms = 0 returns at once;
+ * otherwise the call waits until at least one row is ready or nothing is in
+ * flight, without a limit when timeout_ms < 0, else for at most timeout_ms
+ * milliseconds, after which it returns PGAP_E_TIMEOUT with *n_out = 0 (the
+ * scan goes on; poll again).  An error inside the worker is returned by the
+ * next poll or submit. */
+pgap_status pgap_scan_poll(pgap_engine *e, pgap_scan_row *rows, size_t cap, size_t *n_out, int timeout_ms);
 /* Number of m's submitted and not yet returned. */
 size_t pgap_scan_in_flight(const pgap_engine *e);
 

@@ -27,9 +27,9 @@ static int export_limbs(const mpz_t z, uint32_t *out, int cap) {
     return (int)cnt;
 }
 
-/* Fermat base 2: the test the reference GPU kernel runs with ROUNDS=1
- * (gap_test_gpu.cu:81, miller_rabin.h:157-211), stated textbook-style as
- * mpz_powm(2, N-1, N) == 1. */
+/* Fermat base 2, mpz_powm(2, N-1, N) == 1: the engine's scan filter.  (The
+ * reference GPU kernel with ROUNDS=1, gap_test_gpu.cu:81, runs the stronger
+ * strong base-2 test, sprp_z below with base 2: miller_rabin.h:213-245.) */
 static int fermat2_z(const mpz_t N, mpz_t e, mpz_t r) {
     if (mpz_cmp_ui(N, 1) <= 0) return 0;
     mpz_sub_ui(e, N, 1);

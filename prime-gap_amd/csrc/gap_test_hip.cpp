@@ -63,6 +63,7 @@ struct Args {
     uint64_t mskip = 0;
     int verbose = 2;
     uint32_t depth = 256, confirm = PGAP_CONFIRM_BPSW;
+    int poll_timeout_s = 600;
     size_t hbm_budget = 0;
     std::string results, db;
     int top_percent = 0;
@@ -81,6 +82,7 @@ void usage(const char *name) {
     printf("  --confirm <test>  confirmation of each side's first Fermat PRP: bpsw (default, the\n"
            "                    mpz_probab_prime_p verdict), sprp2 (strong base 2, gap_test_gpu's), none\n");
     printf("  --hbm-budget-mb <n>  device memory the scan may keep resident\n");
+    printf("  --poll-timeout <s>  give up when no row arrives for this long (default 600)\n");
     printf("  --side-skip       gap_test_gpu order: prev side first, skip next if prev merit < min/2-2\n");
     printf("  --results <file>  append result rows (P D m next_p prev_p merit, tab separated)\n");
     printf("  --db <file>       record result / m_stats rows in an SQLite file; skip m's it holds\n");
@@ -106,7 +108,7 @@ Args parse_args(int argc, char **argv) {
         {"confirm", required_argument, 0, 21},           {"hbm-budget-mb", required_argument, 0, 22},
         {"results", required_argument, 0, 23},           {"dump-unknowns", no_argument, 0, 24},
         {"side-skip", no_argument, 0, 25},               {"db", required_argument, 0, 26},
-        {"prp-top-percent", required_argument, 0, 27},
+        {"prp-top-percent", required_argument, 0, 27},  {"poll-timeout", required_argument, 0, 28},
         {"method1", no_argument, 0, 8},                  {"quiet", no_argument, 0, 'q'},
         {"help", no_argument, 0, 'h'},                   {0, 0, 0, 0}};
     Args a;
@@ -141,6 +143,7 @@ Args parse_args(int argc, char **argv) {
             case 25: a.side_skip = true; break;
             case 26: a.db = optarg; break;
             case 27: a.top_percent = atoi(optarg); break;
+            case 28: a.poll_timeout_s = std::max(1, atoi(optarg)); break;
             case 'u':
                 if (!parse_unknown_filename(optarg, a.u, err)) {
                     fprintf(stderr, "%s\n", err.c_str());
@@ -414,7 +417,9 @@ int main(int argc, char **argv) {
     };
     auto drain = [&](bool wait) -> bool {
         size_t n = 0;
-        if (pgap_scan_poll(e, rows.data(), rows.size(), &n, wait ? 1 : 0) != PGAP_OK) {
+        // a scan that returns no row for --poll-timeout seconds is an error
+        // (the worker stalled), not a wait without end
+        if (pgap_scan_poll(e, rows.data(), rows.size(), &n, wait ? a.poll_timeout_s * 1000 : 0) != PGAP_OK) {
             fprintf(stderr, "scan: %s\n", pgap_last_error(e));
             return false;
         }

@@ -48,6 +48,7 @@
 #include <vector>
 
 #include "engine_impl.h"
+#include "waitq.h"
 
 namespace pgap {
 
@@ -1315,17 +1316,20 @@ pgap_status pgap_scan_submit(pgap_engine *e, const pgap_scan_req *req) {
     return PGAP_OK;
 }
 
-pgap_status pgap_scan_poll(pgap_engine *e, pgap_scan_row *rows, size_t cap, size_t *n_out, int wait) {
+pgap_status pgap_scan_poll(pgap_engine *e, pgap_scan_row *rows, size_t cap, size_t *n_out, int timeout_ms) {
     if (!e || !n_out || (cap && !rows)) return PGAP_E_ARG;
     if (!e->ready) return PGAP_E_STATE;
     *n_out = 0;
     Scanner *s = get_scanner(e);
     if (!s->inited) return PGAP_OK;
     std::unique_lock<std::mutex> lk(s->mu);
-    if (wait)
-        s->cv_ready.wait(lk, [&] {
-            return !s->ready.empty() || s->worker_status != PGAP_OK || s->in_flight_locked() == 0;
-        });
+    const bool woke = wait_bounded(s->cv_ready, lk, timeout_ms, [&] {
+        return !s->ready.empty() || s->worker_status != PGAP_OK || s->in_flight_locked() == 0;
+    });
+    if (!woke && timeout_ms > 0) {
+        e->set_error("pgap_scan_poll: no row within %d ms (%zu m's in flight)", timeout_ms, s->in_flight_locked());
+        return PGAP_E_TIMEOUT;
+    }
     if (s->ready.empty() && s->worker_status != PGAP_OK) {
         e->set_error("%s", s->worker_err.c_str());
         return s->worker_status;

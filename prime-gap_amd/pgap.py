@@ -23,6 +23,7 @@ PGAP_E_DEVICE = -2
 PGAP_E_OOM = -3
 PGAP_E_TOO_LARGE = -4
 PGAP_E_STATE = -5
+PGAP_E_TIMEOUT = -6
 
 FERMAT2 = 0
 SPRP = 1
@@ -340,10 +341,15 @@ class Engine:
             self._check(st)
         return st
 
-    def scan_poll(self, cap: int = 4096, wait: bool = True) -> list:
+    def scan_poll(self, cap: int = 4096, wait: bool = True, timeout_ms: int = -1) -> list:
+        """Finished rows.  wait: block until a row is ready or nothing is in
+        flight, at most timeout_ms (< 0: no limit; an empty list when it
+        passes with nothing ready)."""
         rows = (ScanRow * cap)()
         n = ctypes.c_size_t()
-        self._check(self.lib.pgap_scan_poll(self.h, rows, cap, ctypes.byref(n), 1 if wait else 0))
+        st = self.lib.pgap_scan_poll(self.h, rows, cap, ctypes.byref(n), timeout_ms if wait else 0)
+        if st != PGAP_E_TIMEOUT:
+            self._check(st)
         return [rows[i] for i in range(n.value)]
 
     def scan_in_flight(self) -> int:

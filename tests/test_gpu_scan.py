@@ -347,3 +347,21 @@ def test_late_family_wider_than_the_slots_is_refused():
         assert ei.value.status == pgap.PGAP_E_STATE
         while e.scan_in_flight():
             e.scan_poll(wait=True)
+
+
+def test_poll_timeout_then_row():
+    # a 3001#/2190 m takes tens of milliseconds: a 1 ms poll right after the
+    # submit gives PGAP_E_TIMEOUT (an empty list), the row comes later
+    P, D, SL = 3001, 2190, 40000
+    sv = ol.Sieve(P, D, SL, 1_000_000)
+    p, n = sv(1)
+    with pgap.Engine(0) as e:
+        fam = e.family(P, D, SL)
+        assert e.scan_submit(fam, 1, p, n, tag=1) == pgap.PGAP_OK
+        assert e.scan_poll(wait=True, timeout_ms=1) == []
+        assert "no row within 1 ms" in e.last_error()
+        rows = []
+        while e.scan_in_flight():
+            rows += e.scan_poll(wait=True, timeout_ms=60_000)
+    want = ol.test_interval(P, D, 1, SL, p, n, kind=0)
+    assert [(r.m, r.prev_p, r.next_p) for r in rows] == [(1, want["prev_p"], want["next_p"])]
