@@ -170,6 +170,17 @@ pgap_status pgap_filter_candidates(pgap_engine *e, int family, const uint64_t *m
 pgap_status pgap_sieve(pgap_engine *e, int family, const uint64_t *m, size_t count, uint64_t max_prime,
                        int32_t *prev, uint32_t *n_prev, int32_t *next, uint32_t *n_next);
 
+/* Which sides of m*K a request scans.  gap_test.py's probabilistic
+ * side-skip (gap_test.py:169-202) finds the previous prime first and then
+ * decides, from it, whether the next side is worth scanning: the caller
+ * submits the m with PGAP_SIDES_PREV, decides, and submits it again with
+ * PGAP_SIDES_NEXT when it scans on. */
+enum {
+    PGAP_SIDES_BOTH = 0,
+    PGAP_SIDES_PREV = 1,  /* prev side only: row with PGAP_ROW_NEXT_SKIPPED */
+    PGAP_SIDES_NEXT = 2,  /* next side only: row with PGAP_ROW_PREV_SKIPPED */
+};
+
 typedef struct {
     int family;
     uint64_t m;
@@ -178,12 +189,14 @@ typedef struct {
     const int32_t *next;  /* positive distances above m*K, increasing (outward) */
     uint32_t n_next;
     uint64_t tag;         /* caller cookie, echoed in the row */
+    uint32_t sides;       /* PGAP_SIDES_*; 0 (a zeroed request) scans both */
 } pgap_scan_req;
 
 enum {
     PGAP_ROW_PREV_FALLBACK = 1,  /* no PRP among prev unknowns: searched beyond SL */
     PGAP_ROW_NEXT_FALLBACK = 2,
-    PGAP_ROW_NEXT_SKIPPED = 4,  /* side-skip: next side not scanned, next_p = n_tests = 0 */
+    PGAP_ROW_NEXT_SKIPPED = 4,  /* next side not scanned (side-skip or PGAP_SIDES_PREV): next_p = n_tests = 0 */
+    PGAP_ROW_PREV_SKIPPED = 8,  /* prev side not scanned (PGAP_SIDES_NEXT): prev_p = p_tests = 0 */
 };
 
 /* One finished m: the row test_interval_cpu produces (gap_test_common.cpp:

@@ -535,3 +535,93 @@ size_t or_bench_scan(uint32_t P, uint32_t D, uint32_t SL, const uint64_t *m, siz
     free(jobs);
     return count;
 }
+
+/* ---- gap_test.py's probabilistic one-side skip ---------------------------- */
+
+double or_prob_prime_after_sieve(double prob_prime, uint64_t max_prime) {
+    /* gap_test_stats.py:241-250: unknowns_after_sieve = 1 / (ln(max_prime) e^gamma)
+     * (Mertens' third theorem, gamma written as 0.577215665 there) */
+    const double unknowns_after_sieve = 1.0 / (log((double)max_prime) * exp(0.577215665));
+    return prob_prime / unknowns_after_sieve;
+}
+
+static int small_is_prime(uint32_t n) {
+    if (n < 2) return 0;
+    for (uint32_t t = 2; t * t <= n; t++)
+        if (n % t == 0) return 0;
+    return 1;
+}
+
+double or_setup_extended_gap(uint32_t SL, uint32_t P, uint32_t D, double prob_prime, int32_t *xs, int cap,
+                             int *nx) {
+    /* gap_test_stats.py:386-411 */
+    double ppc = prob_prime;
+    for (uint32_t p = 2; p <= P; p++)
+        if (small_is_prime(p)) ppc /= (1 - 1.0 / p);
+    int n = 0;
+    for (uint32_t x = SL + 1; x <= 2 * SL; x++) {
+        int coprime = 1;
+        for (uint32_t p = 2; p <= P && coprime; p++)
+            if (D % p != 0 && x % p == 0 && small_is_prime(p)) coprime = 0;
+        if (coprime) {
+            if (n < cap) xs[n] = (int32_t)x;
+            n++;
+        }
+    }
+    *nx = n;
+    return ppc;
+}
+
+static int is_record(const int64_t *records, size_t n, int64_t megagap, int64_t gap) {
+    if (megagap) return gap > megagap;
+    size_t lo = 0, hi = n;
+    while (lo < hi) {
+        size_t mid = (lo + hi) / 2;
+        if (records[mid] < gap) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n && records[lo] == gap;
+}
+
+static uint64_t gcd_u64(uint64_t a, uint64_t b) {
+    while (b) {
+        uint64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+double or_prob_record_one_sided(const int64_t *records, size_t n_records, int64_t megagap, int64_t other_side,
+                                const int32_t *unknowns_next, size_t n_unknowns, double pps, uint64_t m,
+                                uint64_t K_mod_d, uint64_t d, const int32_t *ext, size_t n_ext, double ppc) {
+    /* gap_test_stats.py:428-453, with its condition on the extension kept as
+     * written: an offset x counts when gcd(m * (K mod d) + x, d) > 1 */
+    double prob_record = 0, prob_nth = 1;
+    for (size_t i = 0; i < n_unknowns; i++) {
+        if (is_record(records, n_records, megagap, unknowns_next[i] + other_side)) prob_record += prob_nth * pps;
+        prob_nth *= 1 - pps;
+    }
+    for (size_t i = 0; i < n_ext; i++) {
+        if (gcd_u64(m * K_mod_d + (uint64_t)ext[i], d) > 1) {
+            if (is_record(records, n_records, megagap, ext[i] + other_side)) prob_record += prob_nth * ppc;
+            prob_nth *= 1 - ppc;
+        }
+    }
+    return prob_record + prob_nth;
+}
+
+void or_side_decisions(size_t n, const double *prob_record, const double *new_prob_record, double prob_threshold,
+                       uint8_t *test_next) {
+    /* gap_test.py:274-275 (start: 5 * prob_threshold over 10 sides) and
+     * :196-202, one m after another */
+    double total = 5 * prob_threshold;
+    long sides = 10;
+    for (size_t i = 0; i < n; i++) {
+        const double threshold = total / sides;
+        const int t = new_prob_record[i] > threshold;
+        total += prob_record[i] - (t ? 0 : new_prob_record[i]);
+        sides += 1 + t;
+        test_next[i] = (uint8_t)t;
+    }
+}

@@ -100,8 +100,26 @@ size_t or_bench_scan(uint32_t P, uint32_t D, uint32_t SL, const uint64_t *m, siz
                      const uint64_t *prev_off, const uint32_t *n_prev, const int32_t *next,
                      const uint64_t *next_off, const uint32_t *n_next, int kind, int threads, int64_t *rows);
 
+/* gap_test.py's probabilistic one-side skip (host policy; the product's
+ * side_prob.h is checked against these):
+ *   or_prob_prime_after_sieve   prob_prime_sieve_length's value (gap_test_stats.py:241-284);
+ *   or_setup_extended_gap       prob_prime_coprime, and the x in (SL, 2SL] coprime to
+ *                               the primes <= P not dividing D (gap_test_stats.py:386-411);
+ *   or_prob_record_one_sided    gap_test_stats.py:428-453 (records sorted ascending;
+ *                               megagap > 0: a gap counts when above it);
+ *   or_side_decisions           the running threshold, gap_test.py:274-275, 196-202. */
+double or_prob_prime_after_sieve(double prob_prime, uint64_t max_prime);
+double or_setup_extended_gap(uint32_t SL, uint32_t P, uint32_t D, double prob_prime, int32_t *xs, int cap,
+                             int *nx);
+double or_prob_record_one_sided(const int64_t *records, size_t n_records, int64_t megagap, int64_t other_side,
+                                const int32_t *unknowns_next, size_t n_unknowns, double pps, uint64_t m,
+                                uint64_t K_mod_d, uint64_t d, const int32_t *ext, size_t n_ext, double ppc);
+void or_side_decisions(size_t n, const double *prob_record, const double *new_prob_record, double prob_threshold,
+                       uint8_t *test_next);
+
 #ifdef __cplusplus
 }
 #endif
+
 
 #endif

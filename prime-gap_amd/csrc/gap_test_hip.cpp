@@ -17,6 +17,13 @@
 // --side-skip selects gap_test_gpu's order instead (gap_test_gpu.cu:591-617):
 // prev side first, and an m whose prev gap has merit below min_merit/2 - 2
 // ends without its next side (counted as "only prev_prime", no row).
+// --side-skip=prob selects gap_test.py's probabilistic one-side skip
+// (gap_test.py:184-202): every m scans its prev side first; then, in m
+// order, the chance of a record given that side alone is set against the
+// running average (side_prob.h), and the next side is scanned only when it
+// is above it.  It reads m_stats.prob_record from --db and the record gaps
+// from --prime-gaps-db (gap_test.py:446-482); a skipped m is saved with
+// next_p = 0, as gap_test.py saves it.
 //
 // Extra flags: --depth N, --confirm bpsw|sprp2|none (scan knobs; sprp2 is
 // the reference GPU tester's strong base-2 verdict), --hbm-budget-mb N,
@@ -42,8 +49,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <fstream>
 #include <map>
+#include <memory>
 #include <set>
 #include <string>
 #include <vector>
@@ -51,6 +60,7 @@
 #include "../../include/pgap.h"
 #include "hostbn.h"
 #include "resultdb.h"
+#include "side_prob.h"
 #include "unknowns.h"
 
 using namespace pgap;
@@ -68,7 +78,10 @@ struct Args {
     std::string results, db;
     int top_percent = 0;
     bool dump = false;
-    bool side_skip = false;
+    bool side_skip = false;       // gap_test_gpu's rule
+    bool side_skip_prob = false;  // gap_test.py's probabilistic rule
+    std::string gaps_db;          // --prime-gaps-db: the record gaps
+    int64_t megagap = 0;
     bool valid = true;
 };
 
@@ -84,6 +97,11 @@ void usage(const char *name) {
     printf("  --hbm-budget-mb <n>  device memory the scan may keep resident\n");
     printf("  --poll-timeout <s>  give up when no row arrives for this long (default 600)\n");
     printf("  --side-skip       gap_test_gpu order: prev side first, skip next if prev merit < min/2-2\n");
+    printf("  --side-skip=prob  gap_test.py's one-side skip: prev side first, next side only when\n"
+           "                    prob(record | prev) beats the running average (needs --db with\n"
+           "                    m_stats.prob_record and --prime-gaps-db)\n");
+    printf("  --prime-gaps-db <file>  record gaps (table gaps: gapsize, merit) for --side-skip=prob\n");
+    printf("  --megagap <n>     with --side-skip=prob: count gaps above n instead of records\n");
     printf("  --results <file>  append result rows (P D m next_p prev_p merit, tab separated)\n");
     printf("  --db <file>       record result / m_stats rows in an SQLite file; skip m's it holds\n");
     printf("  --prp-top-percent <x>  only test the top x%% of m's by m_stats.prob_record (needs --db)\n");
@@ -107,8 +125,9 @@ Args parse_args(int argc, char **argv) {
         {"max-prime", required_argument, 0, 5},          {"depth", required_argument, 0, 20},
         {"confirm", required_argument, 0, 21},           {"hbm-budget-mb", required_argument, 0, 22},
         {"results", required_argument, 0, 23},           {"dump-unknowns", no_argument, 0, 24},
-        {"side-skip", no_argument, 0, 25},               {"db", required_argument, 0, 26},
+        {"side-skip", optional_argument, 0, 25},               {"db", required_argument, 0, 26},
         {"prp-top-percent", required_argument, 0, 27},  {"poll-timeout", required_argument, 0, 28},
+        {"prime-gaps-db", required_argument, 0, 29},     {"megagap", required_argument, 0, 30},
         {"method1", no_argument, 0, 8},                  {"quiet", no_argument, 0, 'q'},
         {"help", no_argument, 0, 'h'},                   {0, 0, 0, 0}};
     Args a;
@@ -140,7 +159,18 @@ Args parse_args(int argc, char **argv) {
             case 22: a.hbm_budget = (size_t)strtoull(optarg, nullptr, 10) << 20; break;
             case 23: a.results = optarg; break;
             case 24: a.dump = true; break;
-            case 25: a.side_skip = true; break;
+            case 25:
+                if (!optarg)
+                    a.side_skip = true;
+                else if (!strcmp(optarg, "prob"))
+                    a.side_skip_prob = true;
+                else {
+                    printf("--side-skip takes no value or =prob: %s\n", optarg);
+                    a.valid = false;
+                }
+                break;
+            case 29: a.gaps_db = optarg; break;
+            case 30: a.megagap = strtoll(optarg, nullptr, 10); break;
             case 26: a.db = optarg; break;
             case 27: a.top_percent = atoi(optarg); break;
             case 28: a.poll_timeout_s = std::max(1, atoi(optarg)); break;
@@ -165,6 +195,14 @@ Args parse_args(int argc, char **argv) {
     if (a.u.sieve_length == 0) { printf("Must set sieve-length\n"); a.valid = false; }
     if (a.top_percent && (a.top_percent < 1 || a.top_percent > 100 || a.db.empty())) {
         printf("--prp-top-percent takes 1..100 and needs --db (m_stats rows from gap_stats)\n");
+        a.valid = false;
+    }
+    if (a.side_skip_prob && (a.db.empty() || a.gaps_db.empty() || a.u.max_prime < 1000000)) {
+        printf("--side-skip=prob needs --db (m_stats rows from gap_stats), --prime-gaps-db and max_prime >= 1M\n");
+        a.valid = false;
+    }
+    if (a.megagap < 0 || (a.megagap && !a.side_skip_prob)) {
+        printf("--megagap takes a positive gap size and goes with --side-skip=prob\n");
         a.valid = false;
     }
     if (a.mskip && (a.mskip < a.u.mstart || a.mskip >= a.u.mstart + a.u.minc)) {
@@ -307,10 +345,15 @@ int main(int argc, char **argv) {
         if (a.verbose >= 0)
             printf("Found %lu results in '%s'\n", (unsigned long)(skip_m.size() + partial.size()), a.db.c_str());
     }
-    if (a.top_percent > 0) {
+    // gap_test.py's m_probs[m][1]: m_stats.prob_record, or prob_merit for
+    // --megagap with --prp-top-percent below 100 (gap_test_stats.py:465-466)
+    std::map<uint64_t, double> m_prob;
+    double prob_threshold = 0;
+    if (a.top_percent > 0 || a.side_skip_prob) {
         // determine_test_threshold (gap_test_stats.py:456-486)
         std::vector<std::pair<uint64_t, float>> probs;
-        if (!db.probs(a.u.p, a.u.d, a.u.mstart, m_hi, probs)) {
+        const bool by_merit = a.megagap && a.top_percent > 0 && a.top_percent < 100;
+        if (!db.probs(a.u.p, a.u.d, a.u.mstart, m_hi, probs, by_merit)) {
             fprintf(stderr, "database '%s': %s (run gap_stats first)\n", a.db.c_str(), db.error().c_str());
             return 1;
         }
@@ -319,7 +362,8 @@ int main(int argc, char **argv) {
                     (unsigned long)probs.size(), (unsigned long)valid_ms);
             return 1;
         }
-        if (a.top_percent < 100) {
+        for (auto &pr : probs) m_prob[pr.first] = pr.second;
+        if (a.top_percent > 0 && a.top_percent < 100) {
             std::vector<float> best;
             for (auto &pr : probs) best.push_back(pr.second);
             std::sort(best.begin(), best.end(), std::greater<float>());
@@ -334,6 +378,39 @@ int main(int argc, char **argv) {
                     skip_m.insert(pr.first);
             }
             if (a.verbose >= 0) printf("Testing %lu m where prob(record) >= %.3g\n", (unsigned long)n_sel, thr);
+            prob_threshold = thr;
+            // a partial row runs whatever its probability, counted at 1.01 x
+            // the threshold (gap_test.py:330-334)
+            for (uint64_t m : partial)
+                if (m_prob.count(m) && m_prob[m] < prob_threshold) m_prob[m] = 1.01 * prob_threshold;
+        }
+    }
+    std::unique_ptr<OneSidedSkip> one_side;
+    if (a.side_skip_prob) {
+        // gap_test.py:446-482, 514-516: records near ln N at mstart, and the
+        // chance that an unknown left by the sieve is prime
+        const double M_log = K_log + log((double)a.u.mstart);
+        std::vector<int64_t> records;
+        std::string err;
+        if (!load_record_gaps(a.gaps_db, M_log, records, err)) {
+            fprintf(stderr, "prime gaps database '%s': %s\n", a.gaps_db.c_str(), err.c_str());
+            return 1;
+        }
+        if (records.empty() && !a.megagap) {
+            fprintf(stderr, "prime gaps database '%s' holds no record gap for ln N = %.2f\n", a.gaps_db.c_str(),
+                    M_log);
+            return 1;
+        }
+        const double prob_prime = side_prob_prime(M_log);
+        const double pps = side_prob_after_sieve(prob_prime, a.u.max_prime);
+        one_side.reset(new OneSidedSkip(records, a.megagap, a.u.p, a.u.d, a.u.sieve_length,
+                                        mod_small(K, a.u.d), prob_prime, pps, prob_threshold));
+        if (a.verbose >= 1) {
+            if (!records.empty())
+                printf("\tLoaded %lu records (%ld to %ld) from '%s'\n", (unsigned long)records.size(),
+                       (long)records.front(), (long)records.back(), a.gaps_db.c_str());
+            if (a.megagap) printf("\tMin merit for megagap(%ld) %.2f\n", (long)a.megagap, a.megagap / M_log);
+            printf("\t~2x%.1f = %.1f PRP tests per m\n", 1 / pps, 2 / pps);
         }
     }
     // the last m this run tests: its row closes the stats
@@ -381,6 +458,59 @@ int main(int argc, char **argv) {
     std::vector<pgap_scan_row> rows(4096);
     int status = 0;
 
+    // --side-skip=prob: m's whose prev side is submitted and whose decision
+    // is pending (decided in m order), prev rows back, next sides decided on
+    // and waiting for admission
+    struct Held {
+        std::vector<int32_t> next;
+        double prob_record;
+    };
+    std::map<uint64_t, Held> undecided;
+    std::map<uint64_t, pgap_scan_row> prev_rows;
+    std::map<uint64_t, std::vector<int32_t>> next_lists;
+    std::deque<uint64_t> outbox;
+
+    auto decide = [&]() {
+        while (!undecided.empty()) {
+            auto it = undecided.begin();
+            auto pr = prev_rows.find(it->first);
+            if (pr == prev_rows.end()) break;
+            const Held &h = it->second;
+            const double p1 = one_side->one_sided(pr->second.prev_p, h.next.data(), h.next.size(), it->first);
+            if (one_side->decide(h.prob_record, p1)) {
+                outbox.push_back(it->first);
+                next_lists[it->first] = std::move(it->second.next);
+            } else {
+                done[it->first] = pr->second;  // flagged PGAP_ROW_NEXT_SKIPPED, next_p = 0
+                prev_rows.erase(pr);
+            }
+            undecided.erase(it);
+        }
+    };
+    // submit decided next sides until admission says the HBM budget is full
+    auto submit_next_sides = [&]() -> bool {
+        while (!outbox.empty()) {
+            const uint64_t m = outbox.front();
+            const std::vector<int32_t> &nx = next_lists[m];
+            pgap_scan_req req{};
+            req.family = fam;
+            req.m = m;
+            req.next = nx.data();
+            req.n_next = (uint32_t)nx.size();
+            req.tag = m;
+            req.sides = PGAP_SIDES_NEXT;
+            const pgap_status st = pgap_scan_submit(e, &req);
+            if (st == PGAP_E_OOM) return true;
+            if (st != PGAP_OK) {
+                fprintf(stderr, "scan submit m=%lu: %s\n", (unsigned long)m, pgap_last_error(e));
+                return false;
+            }
+            next_lists.erase(m);
+            outbox.pop_front();
+        }
+        return true;
+    };
+
     // print finished rows in m order
     auto flush = [&](bool all) {
         while (!pending.empty()) {
@@ -392,9 +522,17 @@ int main(int argc, char **argv) {
             const int prev_p = r.prev_p, next_p = r.next_p, gap = prev_p + next_p;
             const float merit = (float)(gap / (K_log + log((double)m)));
             const bool skipped = (r.flags & PGAP_ROW_NEXT_SKIPPED) != 0;
-            if (skipped) {
-                stats.skips += 1;  // the prev merit stands in (gap_test_gpu.cu:607-613)
-            } else {
+            if (skipped) stats.skips += 1;
+            if (one_side) {
+                // gap_test_stats.py:614-619: a skipped m's row has next_p = 0
+                // and is printed like any other when its merit or record says so
+                const bool record = one_side->is_record(gap);
+                if (record || merit > min_merit)
+                    printf("%-5d %.4f  %ld * %u#/%u -%d to +%d%s\n", gap, merit, m, a.u.p, a.u.d, prev_p, next_p,
+                           record ? "\tRECORD!" : "");
+                if (results)
+                    fprintf(results, "%u\t%u\t%ld\t%d\t%d\t%.4f\n", a.u.p, a.u.d, m, next_p, prev_p, merit);
+            } else if (!skipped) {  // a gap_test_gpu skip prints nothing (gap_test_gpu.cu:607-613)
                 if (merit > min_merit)
                     printf("%-5d %.4f  %ld * %u#/%u -%d to +%d\n", gap, merit, m, a.u.p, a.u.d, prev_p, next_p);
                 if (results)
@@ -423,7 +561,28 @@ int main(int argc, char **argv) {
             fprintf(stderr, "scan: %s\n", pgap_last_error(e));
             return false;
         }
-        for (size_t i = 0; i < n; ++i) done[rows[i].m] = rows[i];
+        for (size_t i = 0; i < n; ++i) {
+            const pgap_scan_row &r = rows[i];
+            if (!one_side) {
+                done[r.m] = r;
+            } else if (r.flags & PGAP_ROW_PREV_SKIPPED) {
+                // the next side of an m whose prev side came back earlier
+                auto pr = prev_rows.find(r.m);
+                pgap_scan_row w = pr->second;
+                w.next_p = r.next_p;
+                w.n_tests = r.n_tests;
+                w.flags = (pr->second.flags & PGAP_ROW_PREV_FALLBACK) | (r.flags & PGAP_ROW_NEXT_FALLBACK);
+                w.t_done = r.t_done;
+                done[r.m] = w;
+                prev_rows.erase(pr);
+            } else {
+                prev_rows[r.m] = r;
+            }
+        }
+        if (one_side) {
+            decide();
+            if (!submit_next_sides()) return false;
+        }
         flush(false);
         return true;
     };
@@ -457,6 +616,13 @@ int main(int argc, char **argv) {
         req.next = next.data();
         req.n_next = (uint32_t)next.size();
         req.tag = m;
+        if (one_side) {
+            req.sides = PGAP_SIDES_PREV;
+            if (!submit_next_sides()) {  // decided next sides go first
+                status = 1;
+                break;
+            }
+        }
         for (;;) {
             pgap_status st = pgap_scan_submit(e, &req);
             if (st == PGAP_OK) break;
@@ -472,8 +638,12 @@ int main(int argc, char **argv) {
         }
         if (status) break;
         pending[m] = Pending{prev.size(), next.size()};
+        if (one_side) {
+            auto pm = m_prob.find(m);
+            undecided[m] = Held{next, pm == m_prob.end() ? 0.0 : pm->second};
+        }
     }
-    while (status == 0 && e && pgap_scan_in_flight(e) > 0)
+    while (status == 0 && e && (pgap_scan_in_flight(e) > 0 || !outbox.empty()))
         if (!drain(true)) status = 1;
     if (status == 0) flush(true);
     if (results) fclose(results);

@@ -11,6 +11,7 @@
 // The image ships libsqlite3.so.0 without its header, so the few C API
 // functions used are declared here (their public, documented signatures).
 #pragma once
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <set>
@@ -22,6 +23,7 @@ extern "C" {
 typedef struct sqlite3 sqlite3;
 typedef struct sqlite3_stmt sqlite3_stmt;
 int sqlite3_open(const char *filename, sqlite3 **db);
+int sqlite3_open_v2(const char *filename, sqlite3 **db, int flags, const char *vfs);
 int sqlite3_close(sqlite3 *db);
 int sqlite3_exec(sqlite3 *db, const char *sql, int (*cb)(void *, int, char **, char **), void *arg, char **err);
 int sqlite3_prepare_v2(sqlite3 *db, const char *sql, int nbyte, sqlite3_stmt **stmt, const char **tail);
@@ -79,12 +81,14 @@ class ResultDb {
 
     // (m, prob_record) of the m_stats rows gap_stats wrote for [lo, hi]
     // (load_probs_only, gap_test_stats.py:170-183), prob_record as float as
-    // the reference's array('f') holds it.  Fails when m_stats has no
-    // prob_record column.
-    bool probs(uint32_t P, uint32_t D, uint64_t lo, uint64_t hi, std::vector<std::pair<uint64_t, float>> &out) {
+    // the reference's array('f') holds it; `merit` reads prob_merit instead.
+    // Fails when m_stats has no such column.
+    bool probs(uint32_t P, uint32_t D, uint64_t lo, uint64_t hi, std::vector<std::pair<uint64_t, float>> &out,
+               bool merit = false) {
         sqlite3_stmt *q = nullptr;
-        if (!prepare("SELECT m, prob_record FROM m_stats WHERE P = ? AND D = ? AND m BETWEEN ? AND ? ORDER BY m", &q))
-            return false;
+        const std::string sql = std::string("SELECT m, ") + (merit ? "prob_merit" : "prob_record") +
+                                " FROM m_stats WHERE P = ? AND D = ? AND m BETWEEN ? AND ? ORDER BY m";
+        if (!prepare(sql.c_str(), &q)) return false;
         bind_range(q, P, D, lo, hi);
         int rc;
         while ((rc = sqlite3_step(q)) == 100 /* SQLITE_ROW */)
@@ -178,5 +182,47 @@ class ResultDb {
         return sqlite3_prepare_v2(db_, sql, -1, s, nullptr) == 0 || fail("prepare");
     }
 };
+
+// The record gap sizes of load_records (gap_test_stats.py:116-142), read
+// from a prime-gap list file (table gaps(gapsize, merit, ...)), opened read
+// only: every gap whose merit at ln N = log_n would beat its record
+// (gapsize > merit * log_n - 0.001) up to merit 35, and, when 30 ln N
+// exceeds 113326 (the smallest gap size missing from the list), every even
+// size from 113326 to 30 ln N absent from the list.  Sorted ascending.
+inline bool load_record_gaps(const std::string &path, double log_n, std::vector<int64_t> &out, std::string &err) {
+    sqlite3 *db = nullptr;
+    auto fail = [&](const char *what) {
+        err = std::string(what) + ": " + (db ? sqlite3_errmsg(db) : "cannot open");
+        if (db) sqlite3_close(db);
+        return false;
+    };
+    if (sqlite3_open_v2(path.c_str(), &db, 0x1 /* SQLITE_OPEN_READONLY */, nullptr) != 0) return fail("open");
+    sqlite3_stmt *q = nullptr;
+    if (sqlite3_prepare_v2(db, "SELECT gapsize FROM gaps WHERE gapsize > merit * ? - 0.001 AND gapsize < 35 * ?",
+                           -1, &q, nullptr) != 0)
+        return fail("select gaps");
+    sqlite3_bind_double(q, 1, log_n);
+    sqlite3_bind_double(q, 2, log_n);
+    int rc;
+    while ((rc = sqlite3_step(q)) == 100) out.push_back(sqlite3_column_int64(q, 0));
+    sqlite3_finalize(q);
+    if (rc != 101) return fail("select gaps");
+    const int64_t smallest_missing = 113326, largest_likely = (int64_t)(log_n * 30);
+    if (smallest_missing < largest_likely) {
+        std::set<int64_t> listed;
+        if (sqlite3_prepare_v2(db, "SELECT gapsize FROM gaps WHERE gapsize BETWEEN ? AND ?", -1, &q, nullptr) != 0)
+            return fail("select gaps");
+        sqlite3_bind_int64(q, 1, smallest_missing);
+        sqlite3_bind_int64(q, 2, largest_likely);
+        while ((rc = sqlite3_step(q)) == 100) listed.insert(sqlite3_column_int64(q, 0));
+        sqlite3_finalize(q);
+        if (rc != 101) return fail("select gaps");
+        for (int64_t g = smallest_missing; g <= largest_likely; g += 2)
+            if (!listed.count(g)) out.push_back(g);
+    }
+    sqlite3_close(db);
+    std::sort(out.begin(), out.end());
+    return true;
+}
 
 }  // namespace pgap

@@ -83,6 +83,7 @@ enum : uint32_t {
     F_PREV_CONF = 64,  // the pending hit's confirmations are issued and not all back
     F_NEXT_CONF = 128,
     F_FINISHED = 256,  // the finished event was written (a second update pass skips the slot)
+    F_PREV_SKIP = 512, // a next-only request: the prev side was not scanned
 };
 
 struct SlotState {
@@ -459,6 +460,7 @@ static uint64_t room_above(const Words &c) {
 struct Staged {
     FamInfo *fam;
     uint64_t m, tag;
+    uint32_t sides;  // PGAP_SIDES_*
     std::vector<int32_t> prev, next;
     double t_submit;
 };
@@ -859,6 +861,9 @@ class Scanner {
                 d.init_flags = F_SEQ;
                 d.skip_below = skip_distance(f.K_log, r.m);
             }
+            // a one-sided request starts with the other side finished
+            if (r.sides == PGAP_SIDES_PREV) d.init_flags |= F_NEXT_DONE | F_NEXT_SKIP;
+            if (r.sides == PGAP_SIDES_NEXT) d.init_flags |= F_PREV_DONE | F_PREV_SKIP;
             d.n0 = sh.len[0];
             d.n1 = sh.len[1];
             d.off_center = (uint32_t)off;
@@ -1153,10 +1158,11 @@ class Scanner {
                 r.family = (uint32_t)sh.fam->id;
                 r.prev_p = ev.x[0];
                 r.next_p = ev.x[1];
-                r.p_tests = std::min(ev.found[0] + 1, sh.orig_len[0]);
+                r.p_tests = ev.found[0] == NONE ? 0 : std::min(ev.found[0] + 1, sh.orig_len[0]);
                 r.n_tests = ev.found[1] == NONE ? 0 : std::min(ev.found[1] + 1, sh.orig_len[1]);
                 r.flags = (sh.fb[0] ? PGAP_ROW_PREV_FALLBACK : 0) | (sh.fb[1] ? PGAP_ROW_NEXT_FALLBACK : 0) |
-                          ((ev.flags & F_NEXT_SKIP) ? PGAP_ROW_NEXT_SKIPPED : 0);
+                          ((ev.flags & F_NEXT_SKIP) ? PGAP_ROW_NEXT_SKIPPED : 0) |
+                          ((ev.flags & F_PREV_SKIP) ? PGAP_ROW_PREV_SKIPPED : 0);
                 r.t_submit = sh.t_submit;
                 r.t_done = t;
                 rows.push_back(r);
@@ -1237,8 +1243,9 @@ pgap_status pgap_scan_submit(pgap_engine *e, const pgap_scan_req *req) {
     if (!e || !req) return PGAP_E_ARG;
     if (!e->ready) return PGAP_E_STATE;
     Family *f = e->family(req->family);
-    if (!f || req->m == 0 || (req->n_prev && !req->prev) || (req->n_next && !req->next)) {
-        e->set_error("pgap_scan_submit: bad family, m or list");
+    if (!f || req->m == 0 || (req->n_prev && !req->prev) || (req->n_next && !req->next) ||
+        req->sides > PGAP_SIDES_NEXT) {
+        e->set_error("pgap_scan_submit: bad family, m, list or sides");
         return PGAP_E_ARG;
     }
     Scanner *s = get_scanner(e);
@@ -1283,8 +1290,10 @@ pgap_status pgap_scan_submit(pgap_engine *e, const pgap_scan_req *req) {
     Staged r;
     r.m = req->m;
     r.tag = req->tag;
-    r.prev.assign(req->prev, req->prev + req->n_prev);
-    r.next.assign(req->next, req->next + req->n_next);
+    r.sides = req->sides;
+    // the side a one-sided request leaves out is neither uploaded nor scanned
+    if (req->sides != PGAP_SIDES_NEXT) r.prev.assign(req->prev, req->prev + req->n_prev);
+    if (req->sides != PGAP_SIDES_PREV) r.next.assign(req->next, req->next + req->n_next);
     r.t_submit = now_s();
     {
         std::lock_guard<std::mutex> g(s->mu);

@@ -37,6 +37,8 @@ CONFIRM_BPSW = 2   # ... confirmed by Baillie-PSW (gap_test_simple's verdict; th
 ROW_PREV_FALLBACK = 1
 ROW_NEXT_FALLBACK = 2
 ROW_NEXT_SKIPPED = 4
+ROW_PREV_SKIPPED = 8
+SIDES_BOTH, SIDES_PREV, SIDES_NEXT = 0, 1, 2
 
 def read_tuning(path: str) -> List[dict]:
     """Parse an autotuner table (the text format pgap_autotune writes)."""
@@ -102,6 +104,7 @@ class ScanReq(ctypes.Structure):
         ("next", ctypes.POINTER(ctypes.c_int32)),
         ("n_next", ctypes.c_uint32),
         ("tag", ctypes.c_uint64),
+        ("sides", ctypes.c_uint32),
     ]
 
 
@@ -330,12 +333,14 @@ class Engine:
         """gap_test_gpu's prev-first order with side-skip below min_merit/2 - 2."""
         self._check(self.lib.pgap_scan_side_skip(self.h, 1 if enable else 0, float(min_merit)))
 
-    def scan_submit(self, family: int, m: int, prev: np.ndarray, nxt: np.ndarray, tag: int = 0) -> int:
-        """Returns PGAP_OK, or PGAP_E_OOM when admission is refused; raises otherwise."""
+    def scan_submit(self, family: int, m: int, prev: np.ndarray, nxt: np.ndarray, tag: int = 0,
+                    sides: int = 0) -> int:
+        """Returns PGAP_OK, or PGAP_E_OOM when admission is refused; raises
+        otherwise.  sides: SIDES_BOTH, SIDES_PREV or SIDES_NEXT."""
         p = np.ascontiguousarray(prev, dtype=np.int32)
         q = np.ascontiguousarray(nxt, dtype=np.int32)
         req = ScanReq(family, m, p.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(p),
-                      q.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(q), tag)
+                      q.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(q), tag, sides)
         st = self.lib.pgap_scan_submit(self.h, ctypes.byref(req))
         if st not in (PGAP_OK, PGAP_E_OOM):
             self._check(st)

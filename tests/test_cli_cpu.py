@@ -174,3 +174,66 @@ def test_db_prp_top_percent_needs_stats(formats, tmp_path):
     # no m_stats rows from gap_stats in a fresh database
     r = run("-u", formats["plain"], "--dump-unknowns", "--db", str(tmp_path / "x.db"), "--prp-top-percent", "10")
     assert r.returncode != 0 and b"gap_stats" in r.stderr
+
+
+def _search_db(path, P, D, ms, prob=0.5):
+    import sqlite3
+    con = sqlite3.connect(path)
+    con.execute("CREATE TABLE m_stats (P INTEGER, D INTEGER, m INTEGER, next_p INTEGER DEFAULT 0, "
+                "prev_p INTEGER DEFAULT 0, merit REAL DEFAULT 0, prob_record REAL, prob_merit REAL, "
+                "prp_next INTEGER DEFAULT 0, prp_prev INTEGER DEFAULT 0, test_time REAL DEFAULT 0, "
+                "PRIMARY KEY(P, D, m))")
+    con.executemany("INSERT INTO m_stats(P, D, m, prob_record, prob_merit) VALUES(?, ?, ?, ?, ?)",
+                    [(P, D, m, prob, prob) for m in ms])
+    con.commit()
+    con.close()
+
+
+def test_prob_side_skip_arguments(formats, tmp_path):
+    r = run("-u", formats["plain"], "--dump-unknowns", "--side-skip=prob")
+    assert r.returncode != 0 and b"needs --db" in r.stdout
+    r = run("-u", formats["plain"], "--dump-unknowns", "--side-skip=sometimes")
+    assert r.returncode != 0 and b"--side-skip takes" in r.stdout
+    r = run("-u", formats["plain"], "--dump-unknowns", "--megagap", "5000")
+    assert r.returncode != 0 and b"--megagap" in r.stdout
+    db = tmp_path / "search.db"
+    _search_db(db, 907, 2190, [m for m in range(1, 200) if m % 2 and m % 3 and m % 5 and m % 73])
+    r = run("-u", formats["plain"], "--dump-unknowns", "--side-skip=prob", "--db", str(db),
+            "--prime-gaps-db", str(tmp_path / "missing.db"))
+    assert r.returncode != 0 and b"prime gaps database" in r.stderr
+
+
+def test_record_gaps_query_and_missing_gaps(tmp_path):
+    # load_records (gap_test_stats.py:116-142): gaps whose merit at ln N
+    # would beat their record, below merit 35; and, once 30 ln N passes
+    # 113326, every even size from there to 30 ln N absent from the list.
+    # 4001#/2190 with m = 1: ln N = 3911.7, so 30 ln N = 117352.
+    import sqlite3
+
+    P, D = 4001, 2190
+    log_n = ol.K_log(P, D)
+    assert 113326 < int(30 * log_n) < 120000
+    unk = tmp_path / f"{P}_{D}_1_3_s100_l100M.txt"
+    unk.write_bytes(uf.plain_line(1, [1, 3], [5, 7]))
+    db = tmp_path / "search.db"
+    _search_db(db, P, D, [1])
+    gaps = tmp_path / "gaps.db"
+    con = sqlite3.connect(gaps)
+    con.execute("CREATE TABLE gaps (gapsize INTEGER, merit REAL)")
+    table = [(10000, 2.0), (10002, 2.6), (20000, 5.0), (20002, 5.2), (136000, 34.0), (137600, 35.1),
+             (113326, 29.0), (113400, 28.9), (117000, 30.0)]
+    con.executemany("INSERT INTO gaps VALUES(?, ?)", table)
+    con.commit()
+    con.close()
+    listed = [g for g, mr in table if g > mr * log_n - 0.001 and g < 35 * log_n]
+    present = {g for g, _ in table if 113326 <= g <= int(30 * log_n)}
+    missing = [g for g in range(113326, int(30 * log_n) + 1, 2) if g not in present]
+    want = sorted(listed + missing)
+    r = run("-u", str(unk), "--dump-unknowns", "--side-skip=prob", "--db", str(db), "--prime-gaps-db", str(gaps))
+    assert r.returncode == 0, r.stderr
+    assert f"Loaded {len(want)} records ({want[0]} to {want[-1]})".encode() in r.stdout
+    # merit 2.0 / 5.0 at ln N 3911.7 beats 10000 / 20000; 2.6 / 5.2 do not;
+    # 137600 is past merit 35; a listed size in the missing range is a
+    # record only by its merit (113400 yes, 113326 and 117000 no)
+    assert {10000, 20000, 136000, 113328, 113400} <= set(want)
+    assert not {10002, 20002, 137600, 113326, 117000} & set(want)

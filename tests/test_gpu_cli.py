@@ -202,3 +202,68 @@ def test_cli_db_side_skip_rows(tmp_path):
             assert row == (w["next_p"], w["prev_p"]) and st == (w["next_p"], w["prev_p"], w["n_tests"]), m
     assert skipped > 0
     con.close()
+
+
+@pytest.mark.parametrize("megagap", [0, 3000])
+def test_cli_probabilistic_side_skip(tmp_path, megagap):
+    # --side-skip=prob: gap_test.py's one-side skip (gap_test.py:184-202,
+    # gap_test_stats.py:428-453) -- prev side first, the next side only when
+    # prob(record | prev gap) beats the running average; skipped m's saved
+    # with next_p = 0.  Decisions from the oracle's restatement, in m order.
+    import sqlite3
+    from test_side_prob import expected_decisions
+
+    P, D, SL = 907, 2190, 11000
+    fixture = _fixture_rows()
+    ms = sorted(fixture)
+    K_log = ol.K_log(P, D)
+    db = tmp_path / "search.db"
+    con = sqlite3.connect(db)
+    con.execute("CREATE TABLE m_stats (P INTEGER, D INTEGER, m INTEGER, next_p INTEGER DEFAULT 0, "
+                "prev_p INTEGER DEFAULT 0, merit REAL DEFAULT 0, prob_record REAL, prob_merit REAL, "
+                "prp_next INTEGER DEFAULT 0, prp_prev INTEGER DEFAULT 0, test_time REAL DEFAULT 0, "
+                "PRIMARY KEY(P, D, m))")
+    probs = [float(np.float32(0.8 + 0.02 * (m % 5))) for m in ms]
+    con.executemany("INSERT INTO m_stats(P, D, m, prob_record, prob_merit) VALUES(?, ?, ?, ?, 0.5)",
+                    [(P, D, m, pr) for m, pr in zip(ms, probs)])
+    con.commit()
+    con.close()
+    gaps = tmp_path / "gaps.db"
+    con = sqlite3.connect(gaps)
+    con.execute("CREATE TABLE gaps (gapsize INTEGER, merit REAL, primedigits INTEGER)")
+    records = list(range(2500, int(35 * K_log), 2))
+    con.executemany("INSERT INTO gaps VALUES(?, 1.0, 300)", [(g,) for g in records])
+    # not records at this size: a merit above what the gap reaches here, and
+    # a gap beyond merit 35
+    con.executemany("INSERT INTO gaps VALUES(?, ?, 300)", [(2000, 5.0), (int(36 * K_log) // 2 * 2, 1.0)])
+    con.commit()
+    con.close()
+
+    cmd = [CLI, "-u", FIXTURE, "--min-merit", "3", "--side-skip=prob", "--db", str(db),
+           "--prime-gaps-db", str(gaps)] + (["--megagap", str(megagap)] if megagap else [])
+    r = subprocess.run(cmd, capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stderr.decode()
+    out = r.stdout.decode()
+    want = {m: ol.test_interval(P, D, m, SL, fixture[m][1], fixture[m][2], kind=0) for m in ms}
+    dec = expected_decisions(P, D, SL, 100_000_000, 1, [(m, want[m]["prev_p"], fixture[m][2]) for m in ms],
+                             [] if megagap else records, megagap, probs)
+    assert 0 < sum(dec) < len(dec)
+    con = sqlite3.connect(db)
+    res = {row[0]: row[1:] for row in con.execute("SELECT m, next_p, prev_p FROM result")}
+    st = {row[0]: row[1:] for row in con.execute("SELECT m, prp_next, prp_prev FROM m_stats")}
+    con.close()
+    rows_out = []
+    for m, t in zip(ms, dec):
+        w = want[m]
+        next_p = w["next_p"] if t else 0
+        assert res[m] == (next_p, w["prev_p"]), m
+        assert st[m] == ((w["n_tests"] if t else 0), w["p_tests"]), m
+        gap = w["prev_p"] + next_p
+        merit = np.float32(gap / (K_log + math.log(m)))
+        rec = gap in set(records)  # the RECORD! tag reads the record list also with --megagap
+        if rec or merit > np.float32(3):
+            rows_out.append("%-5d %.4f  %d * %d#/%d -%d to +%d%s" % (gap, merit, m, P, D, w["prev_p"], next_p,
+                                                                     "\tRECORD!" if rec else ""))
+    printed = [ln for ln in out.splitlines() if re.match(r"^\d+ +\d+\.\d{4}  \d+ \* ", ln)]
+    assert printed == rows_out
+    assert f"only prev_prime {len(ms) - sum(dec)} (" in out

@@ -365,3 +365,37 @@ def test_poll_timeout_then_row():
             rows += e.scan_poll(wait=True, timeout_ms=60_000)
     want = ol.test_interval(P, D, 1, SL, p, n, kind=0)
     assert [(r.m, r.prev_p, r.next_p) for r in rows] == [(1, want["prev_p"], want["next_p"])]
+
+
+def test_one_sided_requests_return_their_half(unknowns_907):
+    # PGAP_SIDES_PREV / PGAP_SIDES_NEXT (the two halves gap_test.py's
+    # one-side skip scans, gap_test.py:169-202): each m submitted three times
+    # (both, prev only, next only) under distinct tags, all in flight together
+    lines = lines_907(unknowns_907)
+    rows = {}
+    with pgap.Engine(0) as e:
+        fam = e.family(907, 2190, 11000)
+        todo = [(m, p, n, s) for m, p, n in lines for s in (pgap.SIDES_BOTH, pgap.SIDES_PREV, pgap.SIDES_NEXT)]
+        while todo or e.scan_in_flight():
+            while todo:
+                m, p, n, s = todo[0]
+                if e.scan_submit(fam, m, np.array(p, np.int32), np.array(n, np.int32), tag=4 * m + s,
+                                 sides=s) == pgap.PGAP_E_OOM:
+                    break
+                todo.pop(0)
+            for r in e.scan_poll(wait=True):
+                rows[r.tag] = r
+    assert len(rows) == 3 * len(lines)
+    for m, p, n in lines:
+        want = ol.test_interval(907, 2190, m, 11000, p, n, kind=0)
+        both, prev, nxt = (rows[4 * m + s] for s in (pgap.SIDES_BOTH, pgap.SIDES_PREV, pgap.SIDES_NEXT))
+        assert (both.prev_p, both.next_p, both.p_tests, both.n_tests) == \
+            (want["prev_p"], want["next_p"], want["p_tests"], want["n_tests"]), m
+        assert (prev.prev_p, prev.next_p, prev.p_tests, prev.n_tests) == (want["prev_p"], 0, want["p_tests"], 0), m
+        assert (nxt.prev_p, nxt.next_p, nxt.p_tests, nxt.n_tests) == (0, want["next_p"], 0, want["n_tests"]), m
+        assert prev.flags == pgap.ROW_NEXT_SKIPPED and nxt.flags == pgap.ROW_PREV_SKIPPED and both.flags == 0
+    with pgap.Engine(0) as e:
+        fam = e.family(907, 2190, 11000)
+        m, p, n = lines[0]
+        with pytest.raises(pgap.PgapError):
+            e.scan_submit(fam, m, np.array(p, np.int32), np.array(n, np.int32), sides=3)
