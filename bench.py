@@ -181,16 +181,25 @@ def sieved_family_rows(engine, hipmem, P2, D2, mlo, mhi, xr, count, rng):
     return rows, bits_hi, e_mean
 
 
-def bench_mixed(engine, hipmem, pgap, count, seed):
-    """Uneven batching: 40% 503#/210, 40% 1511#/2190, 20% 809#/1841070,
-    m ~ U[1, 1e6) coprime to d, x ~ U[-20000, 20000], no factor <= 1e6, in a
-    seeded shuffle.  Compared: one persistent launch over all widths, one
-    launch per width, and every number in the widest width (one container)."""
+MIX_UNEVEN = ((503, 210, 0.4, 1_000_000, 20000), (1511, 2190, 0.4, 1_000_000, 20000),
+              (809, 1841070, 0.2, 1_000_000, 20000))
+# the uneven mix widened to the large-operand family: 3001#/2190 with m < 1e5
+# and x ~ U[-40000, 40000] (4231-4247 bits, eight lanes per number)
+MIX_WIDE = ((503, 210, 0.4, 1_000_000, 20000), (1511, 2190, 0.3, 1_000_000, 20000),
+            (3001, 2190, 0.3, 100_000, 40000))
+
+
+def bench_mixed(engine, hipmem, pgap, count, seed, fams=MIX_UNEVEN):
+    """Uneven batching: by default 40% 503#/210, 40% 1511#/2190, 20%
+    809#/1841070, m ~ U[1, 1e6) coprime to d, x ~ U[-20000, 20000], no factor
+    <= 1e6, in a seeded shuffle.  Compared: one persistent launch over all
+    widths, one launch per width, and every number in the widest width (one
+    container)."""
     rng = np.random.default_rng(seed)
     parts = []
-    for (P2, D2, frac) in ((503, 210, 0.4), (1511, 2190, 0.4), (809, 1841070, 0.2)):
+    for (P2, D2, frac, mhi, xr) in fams:
         n = int(round(count * frac))
-        rows, bits_hi, e_mean = sieved_family_rows(engine, hipmem, P2, D2, 1, 1_000_000, 20000, n, rng)
+        rows, bits_hi, e_mean = sieved_family_rows(engine, hipmem, P2, D2, 1, mhi, xr, n, rng)
         parts.append((P2, rows, bits_hi, e_mean))
     total = sum(len(r) for _, r, _, _ in parts)
     # packed, shuffled
@@ -241,10 +250,10 @@ def bench_mixed(engine, hipmem, pgap, count, seed):
     engine.sync()
     t_one = engine.last_launch().kernel_ms
     v_one = d_out.to_host()
+    desc = ", ".join(f"{100 * fr:.0f}% {P2}#/{D2} ({parts[i][3]:.0f} bits, m~U[1,{mhi:.0e}), x~U[-{xr},{xr}])"
+                     for i, (P2, D2, fr, mhi, xr) in enumerate(fams))
     return {
-        "workload": f"{total} candidates: 40% 503#/210 ({parts[0][3]:.0f} bits), 40% 1511#/2190 "
-                    f"({parts[1][3]:.0f} bits), 20% 809#/1841070 ({parts[2][3]:.0f} bits), m~U[1,1e6), "
-                    f"x~U[-20000,20000], no factor <= 1e6, seeded shuffle",
+        "workload": f"{total} candidates: {desc}, no factor <= 1e6, seeded shuffle",
         "persistent_tests_per_s": total / (t_mixed * 1e-3),
         "per_width_tests_per_s": total / (t_per * 1e-3),
         "one_container_tests_per_s": total / (t_one * 1e-3),
@@ -556,6 +565,7 @@ def main():
         extra["batch_2180_bit"] = bench_batch_2180(engine, hipmem, pgap, 1 << 18, 3, args.seed + 1, peak["mac_per_s"])
     if not args.no_mixed:
         extra["mixed_widths"] = bench_mixed(engine, hipmem, pgap, 1 << 19, args.seed + 3)
+        extra["mixed_widths_with_4247_bit"] = bench_mixed(engine, hipmem, pgap, 1 << 17, args.seed + 4, MIX_WIDE)
     if not args.no_4300:
         extra["batch_4300_bit"] = bench_batch_4300(engine, hipmem, pgap, 1 << 15, args.seed + 4, peak["mac_per_s"])
     if not args.no_plain_503:

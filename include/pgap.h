@@ -108,9 +108,11 @@ pgap_status pgap_prp_batch_device_noalloc(pgap_engine *e, uint32_t limbs, const 
                                           uint8_t *d_verdict, uint32_t max_bits);
 
 /* Mixed widths in one persistent launch.  Number i has limbs[i] words
- * starting at words + offset[i].  Numbers are grouped into size buckets on
- * the device and one work-queue kernel serves every bucket.  FERMAT2 and
- * SPRP only. */
+ * starting at words + offset[i].  Numbers are grouped into size buckets (one
+ * lane per number up to 2180 bits, groups of 4 or 8 lanes up to 4476 bits
+ * for FERMAT2, 4472 for SPRP) and one work-queue kernel serves every bucket.
+ * FERMAT2 and SPRP only.  pgap_last_launch reports lanes_per_number = 0 (it
+ * varies by bucket). */
 pgap_status pgap_prp_batch_mixed(pgap_engine *e, const uint32_t *words, const uint64_t *offset,
                                  const uint32_t *limbs, size_t count, pgap_test kind,
                                  const uint32_t *bases, uint32_t nbases, uint8_t *verdict);

@@ -1,5 +1,6 @@
 // mixed.hip -- pgap_prp_batch_mixed: bucket the numbers, queue tiles widest
-// first and run the persistent kernel of mixed_impl.h.
+// first and run the persistent kernels of mixed_impl.h (lane-group buckets
+// and one-lane buckets side by side on two streams).
 #include <algorithm>
 #include <vector>
 
@@ -73,10 +74,17 @@ static pgap_status mixed_once(pgap_engine *e, const uint32_t *words, const uint6
         e->set_error("pgap_prp_batch_mixed runs FERMAT2 or SPRP (kind %d)", (int)kind);
         return PGAP_E_ARG;
     }
-    // bucket per number: the smallest digit count whose bound covers it
-    constexpr int NB = MixedBuckets::n;
-    // headroom below R = 2^(28 digits): the variants' lazy-reduction bounds
-    const int slack = kind == PGAP_FERMAT2 ? 4 : 8 + (int)(select_bits(KIND_SPRP, 0, bs));
+    // bucket per number: the smallest digit count whose bound covers it, the
+    // one-lane buckets first, then the lane-group ones; headroom below
+    // R = 2^(28 digits): the variants' lazy-reduction bounds (one lane: the
+    // window's; a lane group: 4 or 8 bits)
+    constexpr int N1 = MixedOneLane::n, NB = N1 + MixedGroups::n;
+    const int slack1 = kind == PGAP_FERMAT2 ? 4 : 8 + (int)(select_bits(KIND_SPRP, 0, bs));
+    const int slack_g = kind == PGAP_FERMAT2 ? 4 : 8;
+    auto max_bits = [&](int b) {
+        return b < N1 ? 28 * MixedOneLane::digits[b] - slack1 : 28 * MixedGroups::digits[b - N1] - slack_g;
+    };
+    auto lanes_of = [&](int b) { return b < N1 ? 1 : MixedGroups::lanes[b - N1]; };
     std::vector<std::vector<uint32_t>> by_bucket(NB);
     size_t total_words = 0;
     for (size_t i = 0; i < count; ++i) {
@@ -88,57 +96,93 @@ static pgap_status mixed_once(pgap_engine *e, const uint32_t *words, const uint6
                 break;
             }
         int b = 0;
-        while (b < NB && 28 * MixedBuckets::digits[b] - slack < (int)std::max(bits, 2u)) ++b;
+        while (b < NB && max_bits(b) < (int)std::max(bits, 2u)) ++b;
         if (b == NB) {
             e->set_error("pgap_prp_batch_mixed: number %zu has %u bits, above the widest bucket (%d)", i, bits,
-                         28 * MixedBuckets::digits[NB - 1] - slack);
+                         max_bits(NB - 1));
             return PGAP_E_TOO_LARGE;
         }
         by_bucket[(size_t)b].push_back((uint32_t)i);
         total_words = std::max<size_t>(total_words, offset[i] + limbs[i]);
     }
-    // tiles of 64, widest bucket first
-    std::vector<uint32_t> order;
-    std::vector<uint8_t> tile_bucket;
+    // tiles of one wavefront (64 / lanes numbers), widest bucket first, in
+    // two queues: [0] the lane-group kernel's, [1] the one-lane kernel's
+    std::vector<uint32_t> order[2];
+    std::vector<uint8_t> tile_bucket[2];
     for (int b = NB - 1; b >= 0; --b) {
+        const int q = b < N1 ? 1 : 0;
         const auto &v = by_bucket[(size_t)b];
-        for (size_t k = 0; k < v.size(); k += 64) {
-            tile_bucket.push_back((uint8_t)b);
-            for (size_t l = 0; l < 64; ++l) order.push_back(k + l < v.size() ? v[k + l] : 0xffffffffu);
+        const size_t per = 64 / (size_t)lanes_of(b);
+        for (size_t k = 0; k < v.size(); k += per) {
+            tile_bucket[q].push_back((uint8_t)(b < N1 ? b : b - N1));
+            for (size_t l = 0; l < 64; ++l)
+                order[q].push_back(l < per && k + l < v.size() ? v[k + l] : 0xffffffffu);
         }
     }
-    const uint32_t ntiles = (uint32_t)tile_bucket.size();
+    const uint32_t ntiles[2] = {(uint32_t)tile_bucket[0].size(), (uint32_t)tile_bucket[1].size()};
     (void)hipSetDevice(e->opt.device);
     pgap_status st;
-    const size_t b_words = total_words * 4, b_off = count * 8, b_limbs = count * 4, b_order = order.size() * 4;
-    if ((st = e->ensure(e->d_n, b_words + b_off + b_limbs + b_order + ntiles + 64, "mixed inputs")) ||
-        (st = e->ensure(e->d_out, count, "verdicts")) || (st = e->ensure(e->d_small, 64, "tile counter")))
+    hipError_t he;
+    if (ntiles[0] && ntiles[1] && !e->aux_stream) {
+        if ((he = hipStreamCreateWithFlags(&e->aux_stream, hipStreamNonBlocking)) != hipSuccess ||
+            (he = hipEventCreateWithFlags(&e->ev_aux, hipEventDisableTiming)) != hipSuccess)
+            return e->hip_fail(he, "mixed stream");
+    }
+    const size_t b_words = total_words * 4, b_off = count * 8, b_limbs = count * 4;
+    const size_t b_order = (order[0].size() + order[1].size()) * 4;
+    if ((st = e->ensure(e->d_n, b_words + b_off + b_limbs + b_order + ntiles[0] + ntiles[1] + 64, "mixed inputs")) ||
+        (st = e->ensure(e->d_out, count, "verdicts")) || (st = e->ensure(e->d_small, 64, "tile counters")))
         return st;
     char *base = (char *)e->d_n.p;
     uint32_t *d_words = (uint32_t *)base;
     uint64_t *d_off = (uint64_t *)(base + ((b_words + 7) & ~(size_t)7));
     uint32_t *d_limbs = (uint32_t *)((char *)d_off + b_off);
-    uint32_t *d_order = (uint32_t *)((char *)d_limbs + b_limbs);
-    uint8_t *d_tb = (uint8_t *)((char *)d_order + b_order);
-    hipError_t he;
+    uint32_t *d_order0 = (uint32_t *)((char *)d_limbs + b_limbs);
+    uint32_t *d_order1 = d_order0 + order[0].size();
+    uint8_t *d_tb0 = (uint8_t *)(d_order1 + order[1].size());
+    uint8_t *d_tb1 = d_tb0 + ntiles[0];
+    uint32_t *d_next = (uint32_t *)e->d_small.p;  // [0]: lane-group queue, [1]: one-lane queue
     if ((he = hipMemcpyAsync(d_words, words, b_words, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
         (he = hipMemcpyAsync(d_off, offset, b_off, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
         (he = hipMemcpyAsync(d_limbs, limbs, b_limbs, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
-        (he = hipMemcpyAsync(d_order, order.data(), b_order, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
-        (he = hipMemcpyAsync(d_tb, tile_bucket.data(), ntiles, hipMemcpyHostToDevice, e->stream)) != hipSuccess ||
-        (he = hipMemsetAsync(e->d_small.p, 0, 4, e->stream)) != hipSuccess)
+        (order[0].size() && (he = hipMemcpyAsync(d_order0, order[0].data(), order[0].size() * 4,
+                                                 hipMemcpyHostToDevice, e->stream)) != hipSuccess) ||
+        (order[1].size() && (he = hipMemcpyAsync(d_order1, order[1].data(), order[1].size() * 4,
+                                                 hipMemcpyHostToDevice, e->stream)) != hipSuccess) ||
+        (ntiles[0] && (he = hipMemcpyAsync(d_tb0, tile_bucket[0].data(), ntiles[0], hipMemcpyHostToDevice,
+                                           e->stream)) != hipSuccess) ||
+        (ntiles[1] && (he = hipMemcpyAsync(d_tb1, tile_bucket[1].data(), ntiles[1], hipMemcpyHostToDevice,
+                                           e->stream)) != hipSuccess) ||
+        (he = hipMemsetAsync(d_next, 0, 8, e->stream)) != hipSuccess)
         return e->hip_fail(he, "mixed upload");
-    MixedLaunch ml{d_words, d_off, d_limbs, d_order, d_tb, ntiles, (uint32_t *)e->d_small.p, bs,
-                   (uint8_t *)e->d_out.p};
+    const MixedLaunch lg_launch{d_words, d_off, d_limbs, d_order0, d_tb0, ntiles[0], d_next, bs, (uint8_t *)e->d_out.p};
+    const MixedLaunch one_launch{d_words, d_off, d_limbs, d_order1, d_tb1, ntiles[1], d_next + 1, bs,
+                                 (uint8_t *)e->d_out.p};
+    // the lane-group kernel (the widest tiles) on the engine's stream, the
+    // one-lane kernel beside it on the second stream; the second joins the
+    // first before the end event
     if ((he = hipEventRecord(e->ev0, e->stream)) != hipSuccess) return e->hip_fail(he, "event");
-    he = kind == PGAP_FERMAT2 ? launch_mixed_fermat(ml, e->opt.device, e->stream)
-                              : launch_mixed_sprp(ml, e->opt.device, e->stream);
-    if (he != hipSuccess) return e->hip_fail(he, "mixed kernel");
+    const bool both = ntiles[0] && ntiles[1];
+    if (ntiles[0]) {
+        he = kind == PGAP_FERMAT2 ? launch_mixed_lg_fermat(lg_launch, e->opt.device, e->stream)
+                                  : launch_mixed_lg_sprp(lg_launch, e->opt.device, e->stream);
+        if (he != hipSuccess) return e->hip_fail(he, "mixed lane-group kernel");
+    }
+    if (ntiles[1]) {
+        hipStream_t s1 = both ? e->aux_stream : e->stream;
+        if (both && (he = hipStreamWaitEvent(s1, e->ev0, 0)) != hipSuccess) return e->hip_fail(he, "mixed stream");
+        he = kind == PGAP_FERMAT2 ? launch_mixed_fermat(one_launch, e->opt.device, s1)
+                                  : launch_mixed_sprp(one_launch, e->opt.device, s1);
+        if (he != hipSuccess) return e->hip_fail(he, "mixed kernel");
+        if (both && ((he = hipEventRecord(e->ev_aux, s1)) != hipSuccess ||
+                     (he = hipStreamWaitEvent(e->stream, e->ev_aux, 0)) != hipSuccess))
+            return e->hip_fail(he, "mixed stream");
+    }
     if ((he = hipEventRecord(e->ev1, e->stream)) != hipSuccess) return e->hip_fail(he, "event");
     e->last.digits = 0;
     e->last.radix_bits = 28;
-    e->last.lanes_per_number = 1;
-    e->last.launches = 1;
+    e->last.lanes_per_number = 0;  // per bucket
+    e->last.launches = (ntiles[0] ? 1u : 0u) + (ntiles[1] ? 1u : 0u);
     e->last.tests = count;
     e->timing_pending = true;
     if ((he = hipMemcpyAsync(verdict, e->d_out.p, count, hipMemcpyDeviceToHost, e->stream)) != hipSuccess ||

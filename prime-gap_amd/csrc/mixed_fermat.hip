@@ -1,4 +1,4 @@
-// Instantiation unit: persistent mixed-width kernel, Fermat base 2.
+// Instantiation unit: persistent mixed-width kernel, one-lane buckets, Fermat base 2.
 #include "mixed_impl.h"
 namespace pgap {
 hipError_t launch_mixed_fermat(const MixedLaunch &a, int device, hipStream_t stream) {

@@ -1,4 +1,4 @@
-// Instantiation unit: persistent mixed-width kernel, strong PRP.
+// Instantiation unit: persistent mixed-width kernel, one-lane buckets, strong PRP.
 #include "mixed_impl.h"
 namespace pgap {
 hipError_t launch_mixed_sprp(const MixedLaunch &a, int device, hipStream_t stream) {

@@ -57,6 +57,65 @@ def test_mixed_sprp(engine):
 
 
 def test_mixed_rejects_too_wide(engine):
+    # the widest bucket: 8 lanes x 20 digits of 28 bits, less 4 bits of
+    # headroom for Fermat, 8 for the strong test
+    engine.prp_mixed(limb_rows([3, (1 << 4475) + 1]))
     with pytest.raises(pgap.PgapError) as ei:
-        engine.prp_mixed(limb_rows([3, (1 << 3000) + 1]))
+        engine.prp_mixed(limb_rows([3, (1 << 4476) + 1]))
     assert ei.value.status == pgap.PGAP_E_TOO_LARGE
+    engine.prp_mixed(limb_rows([3, (1 << 4471) + 1]), pgap.SPRP, (2,))
+    with pytest.raises(pgap.PgapError) as ei:
+        engine.prp_mixed(limb_rows([3, (1 << 4472) + 1]), pgap.SPRP, (2,))
+    assert ei.value.status == pgap.PGAP_E_TOO_LARGE
+
+
+# Mersenne primes across the lane-group buckets (2^p - 1 for p = 2203, 2281,
+# 3217, 4253, 4423), composites next to them, and candidates of the large
+# families (1511#/2190: 2128 bits; 3001#/2190: 4231-4247 bits)
+MERSENNE = [2203, 2281, 3217, 4253, 4423]
+
+
+def wide_mix(count, seed):
+    rng = random.Random(seed)
+    vals = []
+    K15, K30 = ol.K(1511, 2190), ol.K(3001, 2190)
+    for bits in (2182, 2500, 2684, 2700, 3000, 3356, 3400, 3580, 3600, 4000, 4252, 4300, 4476):
+        for _ in range(8):
+            vals.append(rng.getrandbits(bits) | (1 << (bits - 1)) | 1)
+    while len(vals) < count:
+        m = rng.randrange(1, 100_000)
+        if math.gcd(m, 2190) != 1:
+            continue
+        K = K30 if rng.random() < 0.6 else K15
+        vals.append(m * K + rng.randrange(-40000, 40001))
+    for p in MERSENNE:
+        vals += [(1 << p) - 1, (1 << p) + 1, ((1 << p) - 1) * 3]
+    return vals
+
+
+@pytest.mark.parametrize("kind,bases", [(pgap.FERMAT2, ()), (pgap.SPRP, (2, 3))])
+def test_mixed_lane_group_buckets(engine, kind, bases):
+    vals = wide_mix(700, 23) + mix(300, 24)
+    if kind == pgap.SPRP:
+        vals = [v for v in vals if v.bit_length() <= 4472]
+    rows = limb_rows(vals)
+    got = engine.prp_mixed(rows, kind, bases) if bases else engine.prp_mixed(rows)
+    # the waves of one workgroup run buckets of different widths at once:
+    # repeated launches place the tiles differently
+    for _ in range(3):
+        again = engine.prp_mixed(rows, kind, bases) if bases else engine.prp_mixed(rows)
+        assert np.array_equal(again, got)
+    okind = 0 if kind == pgap.FERMAT2 else 1
+    for width in sorted(set(len(r) for r in rows)):
+        idx = [i for i, r in enumerate(rows) if len(r) == width]
+        arr = np.stack([rows[i] for i in idx])
+        want = ol.verdicts(arr, okind, bases) if bases else ol.verdicts(arr, okind)
+        assert np.array_equal(got[idx], want), width
+    for p in MERSENNE:
+        i = vals.index((1 << p) - 1)
+        assert got[i] == 1 and got[i + 2] == 0, p
+    # the same numbers in the per-width batch path
+    wide = [i for i, v in enumerate(vals) if v.bit_length() > 4000]
+    arr = np.stack([pgap.to_limbs([vals[i]], 140)[0] for i in wide])
+    per = engine.prp(arr, kind, bases) if bases else engine.prp(arr)
+    assert np.array_equal(got[wide], per)
