@@ -480,11 +480,6 @@ void pgap_destroy(pgap_engine *e) {
         }
         (void)hipEventDestroy(e->ev0);
         (void)hipEventDestroy(e->ev1);
-        if (e->ev_aux) (void)hipEventDestroy(e->ev_aux);
-        if (e->aux_stream) {
-            (void)hipStreamSynchronize(e->aux_stream);
-            (void)hipStreamDestroy(e->aux_stream);
-        }
         (void)hipStreamDestroy(e->stream);
     }
     delete e;

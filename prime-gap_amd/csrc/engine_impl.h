@@ -92,10 +92,6 @@ struct pgap_engine {
     pgap::Choice select(int kind, uint32_t bits) const;             // variant + occupancy cap, or none
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // a second stream for the mixed batch's one-lane kernel, which runs
-    // beside its lane-group kernel; created at the first mixed batch
-    hipStream_t aux_stream = nullptr;
-    hipEvent_t ev_aux = nullptr;
     bool timing_pending = false;
     pgap_launch_info last{};
 

@@ -123,9 +123,25 @@ static pgap_status mixed_once(pgap_engine *e, const uint32_t *words, const uint6
     (void)hipSetDevice(e->opt.device);
     pgap_status st;
     hipError_t he;
-    if (ntiles[0] && ntiles[1] && !e->aux_stream) {
-        if ((he = hipStreamCreateWithFlags(&e->aux_stream, hipStreamNonBlocking)) != hipSuccess ||
-            (he = hipEventCreateWithFlags(&e->ev_aux, hipEventDisableTiming)) != hipSuccess)
+    // The one-lane kernel runs beside the lane-group kernel on a stream of
+    // its own, created for this call and destroyed at its end: a process
+    // gets a few hardware queues (GPU_MAX_HW_QUEUES, 4 by default), and a
+    // stream kept alive would make the streaming scan's two streams share
+    // one with another (measured: 9.8 k -> 6.7 k m/s at 1511#).
+    struct AuxStream {
+        hipStream_t s = nullptr;
+        hipEvent_t ev = nullptr;
+        ~AuxStream() {
+            if (s) {
+                (void)hipStreamSynchronize(s);
+                (void)hipStreamDestroy(s);
+            }
+            if (ev) (void)hipEventDestroy(ev);
+        }
+    } aux;
+    if (ntiles[0] && ntiles[1]) {
+        if ((he = hipStreamCreateWithFlags(&aux.s, hipStreamNonBlocking)) != hipSuccess ||
+            (he = hipEventCreateWithFlags(&aux.ev, hipEventDisableTiming)) != hipSuccess)
             return e->hip_fail(he, "mixed stream");
     }
     const size_t b_words = total_words * 4, b_off = count * 8, b_limbs = count * 4;
@@ -169,13 +185,13 @@ static pgap_status mixed_once(pgap_engine *e, const uint32_t *words, const uint6
         if (he != hipSuccess) return e->hip_fail(he, "mixed lane-group kernel");
     }
     if (ntiles[1]) {
-        hipStream_t s1 = both ? e->aux_stream : e->stream;
+        hipStream_t s1 = both ? aux.s : e->stream;
         if (both && (he = hipStreamWaitEvent(s1, e->ev0, 0)) != hipSuccess) return e->hip_fail(he, "mixed stream");
         he = kind == PGAP_FERMAT2 ? launch_mixed_fermat(one_launch, e->opt.device, s1)
                                   : launch_mixed_sprp(one_launch, e->opt.device, s1);
         if (he != hipSuccess) return e->hip_fail(he, "mixed kernel");
-        if (both && ((he = hipEventRecord(e->ev_aux, s1)) != hipSuccess ||
-                     (he = hipStreamWaitEvent(e->stream, e->ev_aux, 0)) != hipSuccess))
+        if (both && ((he = hipEventRecord(aux.ev, s1)) != hipSuccess ||
+                     (he = hipStreamWaitEvent(e->stream, aux.ev, 0)) != hipSuccess))
             return e->hip_fail(he, "mixed stream");
     }
     if ((he = hipEventRecord(e->ev1, e->stream)) != hipSuccess) return e->hip_fail(he, "event");
