@@ -22,7 +22,7 @@
 // order, the chance of a record given that side alone is set against the
 // running average (side_prob.h), and the next side is scanned only when it
 // is above it.  It reads m_stats.prob_record from --db and the record gaps
-// from --prime-gaps-db (gap_test.py:446-482); a skipped m is saved with
+// from --prime-gaps-db (gap_test.py:465-486); a skipped m is saved with
 // next_p = 0, as gap_test.py saves it.
 //
 // Extra flags: --depth N, --confirm bpsw|sprp2|none (scan knobs; sprp2 is
@@ -380,14 +380,14 @@ int main(int argc, char **argv) {
             if (a.verbose >= 0) printf("Testing %lu m where prob(record) >= %.3g\n", (unsigned long)n_sel, thr);
             prob_threshold = thr;
             // a partial row runs whatever its probability, counted at 1.01 x
-            // the threshold (gap_test.py:330-334)
+            // the threshold (gap_test.py:332-336)
             for (uint64_t m : partial)
                 if (m_prob.count(m) && m_prob[m] < prob_threshold) m_prob[m] = 1.01 * prob_threshold;
         }
     }
     std::unique_ptr<OneSidedSkip> one_side;
     if (a.side_skip_prob) {
-        // gap_test.py:446-482, 514-516: records near ln N at mstart, and the
+        // gap_test.py:465-486, 514-516: records near ln N at mstart, and the
         // chance that an unknown left by the sieve is prime
         const double M_log = K_log + log((double)a.u.mstart);
         std::vector<int64_t> records;

@@ -66,7 +66,7 @@ class ResultDb {
 
     // `result` rows of (P, D) with m in [lo, hi]: finished ones (next_p >= 0;
     // 0 marks a side-skipped m) into `done`, partial ones (next_p < 0, tested
-    // again whatever their probability -- gap_test.py:330-349) into `partial`.
+    // again whatever their probability -- gap_test.py:331-350) into `partial`.
     bool existing(uint32_t P, uint32_t D, uint64_t lo, uint64_t hi, std::set<uint64_t> &done,
                   std::set<uint64_t> &partial) {
         sqlite3_stmt *q = nullptr;

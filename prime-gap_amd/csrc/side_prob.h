@@ -21,7 +21,7 @@ namespace pgap {
 // The Euler-Mascheroni constant as gap_test_stats.py:245 writes it.
 constexpr double kSideSkipGamma = 0.577215665;
 
-// prob_prime = 1/ln N - 1/ln^2 N with ln N taken at mstart (gap_test.py:514).
+// prob_prime = 1/ln N - 1/ln^2 N with ln N taken at mstart (gap_test.py:515).
 inline double side_prob_prime(double log_n) { return 1 / log_n - 1 / (log_n * log_n); }
 
 // The return value of prob_prime_sieve_length (gap_test_stats.py:241-284):

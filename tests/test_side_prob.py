@@ -198,7 +198,7 @@ def test_product_equals_oracle(P, D, SL, megagap):
 def expected_decisions(P, D, SL, max_prime, mstart, rows, records, megagap, prob_record, prob_threshold=0.0):
     """The oracle's one-side decisions for m's in order: rows = [(m, prev_p,
     next unknowns)], prob_record[i] = m_stats' value for rows[i]; ln N taken
-    at mstart (gap_test.py:514)."""
+    at mstart (gap_test.py:515)."""
     M_log = ol.K_log(P, D) + math.log(mstart)
     pp = 1 / M_log - 1 / (M_log * M_log)
     pps = oracle().or_prob_prime_after_sieve(pp, max_prime)
