@@ -62,10 +62,12 @@ constexpr uint32_t MAX_SLOTS = 1u << 16;  // m's in flight (the HBM budget may a
 // (2128 bits, 2048 numbers: 11.7 against 40.2 ms) at 59% of its throughput.
 // While at most LATENCY_SIDES sides scan, rounds run on the 8-lane variants
 // (when the one-lane variant is wide): many short rounds, each side a
-// shallower window.  1511#/2190 at half of capacity: p50/p99 per m 50/139 ms
-// against 126/206 with one-lane rounds; at 0.8 of capacity 144/316 against
-// 142/297 (above the threshold the one-lane rounds take over again).
-constexpr uint32_t LATENCY_SIDES = 2048;     // scanning sides at most
+// shallower window; above the threshold the one-lane rounds take over
+// again.  1511#/2190 with Baillie-PSW confirmation (p50/p99 per m, ms):
+// threshold 1024: 78/138 at 0.3 of capacity, 118/332 at 4900 m/s, 169/265
+// at half of capacity (5.9 k m/s); 2048: 64/124, 123/364, 193/559; 640:
+// 80/139, 164/229, 168/238.
+constexpr uint32_t LATENCY_SIDES = 1024;     // scanning sides at most
 constexpr uint32_t LATENCY_SIDES_ROW = 256;  // ... and at most this many for 16 lanes instead of 8
 constexpr int LATENCY_MIN_DIGITS = 52;       // one-lane variant at least this wide
 constexpr uint32_t LUCAS_POOLS = 4;  // Lucas batches that may run at once (a ring)
@@ -706,7 +708,8 @@ class Scanner {
         if ((he = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess ||
             (he = hipStreamCreateWithFlags(&lstream, hipStreamNonBlocking)) != hipSuccess ||
             (he = hipEventCreateWithFlags(&ev_rows, hipEventDisableTiming)) != hipSuccess ||
-            (d_lctr && (he = hipMemset(d_lctr, 0, 64 * sizeof(uint32_t))) != hipSuccess)) {
+            (d_lctr && ((he = hipMemsetAsync(d_lctr, 0, 64 * sizeof(uint32_t), stream)) != hipSuccess ||
+                        (he = hipStreamSynchronize(stream)) != hipSuccess))) {
             release();
             return e->hip_fail(he, "scan streams");
         }
@@ -1114,7 +1117,9 @@ class Scanner {
             }
         }
         if (nev) {
-            if ((he = hipMemcpy(h_events, d_events, nev * sizeof(Event), hipMemcpyDeviceToHost)) != hipSuccess)
+            if ((he = hipMemcpyAsync(h_events, d_events, nev * sizeof(Event), hipMemcpyDeviceToHost, stream)) !=
+                    hipSuccess ||
+                (he = hipStreamSynchronize(stream)) != hipSuccess)
                 return e->hip_fail(he, "scan events");
         }
         const double t = now_s();
