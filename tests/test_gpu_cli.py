@@ -204,8 +204,8 @@ def test_cli_db_side_skip_rows(tmp_path):
     con.close()
 
 
-@pytest.mark.parametrize("megagap", [0, 3000])
-def test_cli_probabilistic_side_skip(tmp_path, megagap):
+@pytest.mark.parametrize("megagap,top", [(0, 0), (3000, 0), (0, 60)])
+def test_cli_probabilistic_side_skip(tmp_path, megagap, top):
     # --side-skip=prob: gap_test.py's one-side skip (gap_test.py:184-202,
     # gap_test_stats.py:428-453) -- prev side first, the next side only when
     # prob(record | prev gap) beats the running average; skipped m's saved
@@ -240,16 +240,27 @@ def test_cli_probabilistic_side_skip(tmp_path, megagap):
     con.close()
 
     cmd = [CLI, "-u", FIXTURE, "--min-merit", "3", "--side-skip=prob", "--db", str(db),
-           "--prime-gaps-db", str(gaps)] + (["--megagap", str(megagap)] if megagap else [])
+           "--prime-gaps-db", str(gaps)] + (["--megagap", str(megagap)] if megagap else []) + \
+        (["--prp-top-percent", str(top)] if top else [])
     r = subprocess.run(cmd, capture_output=True, timeout=300)
     assert r.returncode == 0, r.stderr.decode()
     out = r.stdout.decode()
+    # --prp-top-percent (determine_test_threshold, gap_test_stats.py:456-486):
+    # the m's at or above the prob_record at index round(n * top / 100) of
+    # the descending list; the one-side threshold starts at 5 x that value
+    thr = 0.0
+    if top:
+        thr = sorted(probs, reverse=True)[round(len(probs) * top / 100)]
+        keep = [i for i, pr in enumerate(probs) if pr >= thr]
+        ms, probs = [ms[i] for i in keep], [probs[i] for i in keep]
+        assert 0 < len(ms) < len(fixture)
     want = {m: ol.test_interval(P, D, m, SL, fixture[m][1], fixture[m][2], kind=0) for m in ms}
     dec = expected_decisions(P, D, SL, 100_000_000, 1, [(m, want[m]["prev_p"], fixture[m][2]) for m in ms],
-                             [] if megagap else records, megagap, probs)
+                             [] if megagap else records, megagap, probs, thr)
     assert 0 < sum(dec) < len(dec)
     con = sqlite3.connect(db)
     res = {row[0]: row[1:] for row in con.execute("SELECT m, next_p, prev_p FROM result")}
+    assert sorted(res) == ms
     st = {row[0]: row[1:] for row in con.execute("SELECT m, prp_next, prp_prev FROM m_stats")}
     con.close()
     rows_out = []
