@@ -40,6 +40,7 @@ ROW_NEXT_SKIPPED = 4
 ROW_PREV_SKIPPED = 8
 SIDES_BOTH, SIDES_PREV, SIDES_NEXT = 0, 1, 2
 
+
 def read_tuning(path: str) -> List[dict]:
     """Parse an autotuner table (the text format pgap_autotune writes)."""
     keys = ("test", "bits", "radix", "digits", "lanes", "cap", "tests_per_s")
