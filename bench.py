@@ -262,11 +262,12 @@ def bench_mixed(engine, hipmem, pgap, count, seed, fams=MIX_UNEVEN):
     }
 
 
-def bench_batch_4300(engine, hipmem, pgap, count, seed, peak_mac):
-    """Large operands: N = m*3001#/2190 + x, m ~ U[1, 1e5), x ~ U[-40000,
-    40000], no factor <= 1e6 (4231-4248 bits, quad-per-number variants)."""
+def bench_batch_family(engine, hipmem, pgap, P2, D2, m_hi, xr, count, seed, peak_mac, container_limbs):
+    """A fixed-width Fermat batch of one family: N = m*P2#/D2 + x, m ~ U[1,
+    m_hi), x ~ U[-xr, xr], no prime factor <= 1e6; one warm-up launch and
+    two timed ones."""
     rng = np.random.default_rng(seed)
-    rows, bits_hi, e_mean = sieved_family_rows(engine, hipmem, 3001, 2190, 1, 100_000, 40000, count, rng)
+    rows, bits_hi, e_mean = sieved_family_rows(engine, hipmem, P2, D2, 1, m_hi, xr, count, rng)
     d_n = hipmem.DeviceArray.from_host(rows)
     d_o = hipmem.DeviceArray(count, np.uint8)
     ks = []
@@ -279,16 +280,30 @@ def bench_batch_4300(engine, hipmem, pgap, count, seed, peak_mac):
     avg = float(np.mean(ks)) * 1e-3
     L32 = (int(e_mean + 0.999) + 31) // 32
     return {
-        "workload": f"N = m*3001#/2190 + x, m~U[1,1e5), x~U[-40000,40000], no factor <= 1e6, {count} "
+        "workload": f"N = m*{P2}#/{D2} + x, m~U[1,{m_hi:.0e}), x~U[-{xr},{xr}], no factor <= 1e6, {count} "
                     f"candidates, {e_mean:.0f} bits",
         "tests_per_s": count / avg,
         "kernel_ms_avg": avg * 1e3,
         "kernel": f"radix-2^{info.radix_bits} x {info.digits} digits, {info.lanes_per_number} lanes/number",
         "roofline_frac": 2 * L32 * L32 * e_mean * count / avg / peak_mac,
         "roofline_frac_of_nominal": 2 * L32 * L32 * e_mean * count / avg / PEAK_INT32_MAC,
-        "roofline_frac_container_136_limbs": 2 * 136 * 136 * e_mean * count / avg / peak_mac,
+        f"roofline_frac_container_{container_limbs}_limbs":
+            2 * container_limbs * container_limbs * e_mean * count / avg / peak_mac,
         "work_per_test": f"2*L^2*e, L={L32}, e={e_mean:.1f}",
     }
+
+
+def bench_batch_4300(engine, hipmem, pgap, count, seed, peak_mac):
+    """Large operands: N = m*3001#/2190 + x, m ~ U[1, 1e5), x ~ U[-40000,
+    40000], no factor <= 1e6 (4231-4248 bits, eight lanes per number)."""
+    return bench_batch_family(engine, hipmem, pgap, 3001, 2190, 100_000, 40000, count, seed, peak_mac, 136)
+
+
+def bench_batch_700(engine, hipmem, pgap, count, seed, peak_mac):
+    """Narrow operands, the 503#/210 family: N = m*503#/210 + x, m ~ U[1,
+    1e6), x ~ U[-20000, 20000], no factor <= 1e6 (about 705 bits; a
+    container of 24 32-bit limbs)."""
+    return bench_batch_family(engine, hipmem, pgap, 503, 210, 1_000_000, 20000, count, seed, peak_mac, 24)
 
 
 def bench_stream(engine, pgap, n_m, load, seed, cpu_sample, threads, P4=1511, D4=2190, SL4=40000, m_hi=200000,
@@ -417,6 +432,7 @@ def main():
     ap.add_argument("--no-plain-503", action="store_true", help="skip the plain 503#/210 file scan")
     ap.add_argument("--mad-peak", action="store_true", help="only measure the v_mad_u64_u32 rate")
     ap.add_argument("--no-4300", action="store_true", help="skip the 4300-bit batch")
+    ap.add_argument("--no-700", action="store_true", help="skip the 700-bit batch")
     ap.add_argument("--stream-m", type=int, default=16384)
     ap.add_argument("--stream-load", type=float, default=0.5)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -566,6 +582,8 @@ def main():
     if not args.no_mixed:
         extra["mixed_widths"] = bench_mixed(engine, hipmem, pgap, 1 << 19, args.seed + 3)
         extra["mixed_widths_with_4247_bit"] = bench_mixed(engine, hipmem, pgap, 1 << 17, args.seed + 4, MIX_WIDE)
+    if not args.no_700:
+        extra["batch_700_bit"] = bench_batch_700(engine, hipmem, pgap, 1 << 20, args.seed + 7, peak["mac_per_s"])
     if not args.no_4300:
         extra["batch_4300_bit"] = bench_batch_4300(engine, hipmem, pgap, 1 << 15, args.seed + 4, peak["mac_per_s"])
     if not args.no_plain_503:
