@@ -19,11 +19,13 @@
 // 28-bit digit: one 32-bit DPP move per row (see sqr_redc for the bound).
 // The S rows that multiply by one lane's digits share one unrolled body.
 //
-// G = 8 halves the registers per lane (5S + temporaries), which lets a
-// 4247-bit number run at more than one wavefront per SIMD without the
-// accumulation registers spilling to AGPRs.  G = 16 (a whole DPP row) halves
-// a test's latency again at a lower throughput; the streaming scan uses it
-// while few m's are in flight.
+// A 4247-bit number (152 digits) runs on G = 4 lanes of S = 38 digits at
+// two wavefronts per SIMD (x, N and the accumulators take 152 registers per
+// lane; sqr_redc keeps its rows in order, see there), or on G = 8 lanes of
+// 19 digits at three.  The quad issues 14% fewer instructions per test (one
+// m broadcast per 76 products instead of per 38) and is the faster of the
+// two.  G = 16 (a whole DPP row) cuts a test's latency at a lower
+// throughput; the streaming scan uses it while few m's are in flight.
 //
 // Reference counterpart: CGBN's warp-cooperative cgbn_mont_sqr with
 // TPI threads per instance (miller_rabin.h:157-211, TPI = 8 in
@@ -293,25 +295,39 @@ __device__ __forceinline__ void canon(uint32_t (&x)[S], uint32_t g) {
     normalize<S, G>(t, x, g);
 }
 
+// One row of the squaring at rotation u: T += x_i * x; m from the bottom
+// digit (lane 0); T += m * N; then every lane adds its bottom accumulator's
+// bits above 28 to its next position and hands the low 28 bits to the lane
+// below; the group's top lane takes zero.
+template <int S, int G>
+__device__ __forceinline__ void cios_row(uint64_t (&t)[S], const uint32_t (&x)[S], const uint32_t (&n)[S],
+                                         uint32_t xi, uint32_t np, bool top, int u) {
+#pragma unroll
+    for (int j = 0; j < S; ++j) t[(u + j) % S] = mad(xi, x[j], t[(u + j) % S]);
+    const uint32_t m = gbcast0<G>(((uint32_t)t[u] * np) & MASK);
+#pragma unroll
+    for (int j = 0; j < S; ++j) t[(u + j) % S] = mad(m, n[j], t[(u + j) % S]);
+    t[(u + 1) % S] += t[u] >> RB;
+    const uint32_t lo = row_down((uint32_t)t[u] & MASK);
+    t[u] = top ? 0u : lo;
+}
+
 // x <- x^2 R^{-1} mod N (lazy: x < 4N in, < 2N out), R = 2^(28*G*S).
 // `stage` is the group's LDS stage (G rows of stage_stride(S) words).
 // LAZY: the result comes as lazy_carry digits times 2^s (< 4N); otherwise as
 // exact digits (s must be 0).
 //
-// One row: T += x_i * x; m from the bottom digit (lane 0); T += m * N; then
-// every lane adds its bottom accumulator's bits above 28 to its next
-// position and hands the low 28 bits to the lane below; the group's top lane
-// takes zero.  Lane 0's low bits are zero (that is what m is for), so
-// T /= 2^28 is exact.  Accumulator bound: a value enters a lane below 2^28,
-// takes two products (< 2^56 each) per row for at most S rows and one carry
-// below 2^36, and leaves the lane at 28 bits again: 2S(2^28 - 1)^2 + 2^36 +
-// 2^28 < 2^64 for S <= 127, whatever G, with exact input digits.  With
-// lazy_carry digits (each product < 2^58) the same count gives
-// S(2^58 + 2^56) + 2^37 < 2^64 for S <= 32.
+// Rows as in cios_row.  Lane 0's low bits are zero after each row (that is
+// what m is for), so T /= 2^28 is exact.  Accumulator bound: a value enters
+// a lane below 2^28, takes two products (< 2^56 each) per row for at most S
+// rows and one carry below 2^36, and leaves the lane at 28 bits again:
+// 2S(2^28 - 1)^2 + 2^36 + 2^28 < 2^64 for S <= 127, whatever G, with exact
+// input digits.  With lazy_carry digits (x_i * x_j < 2^58, m * n_j < 2^56)
+// the same count gives S(2^58 + 2^56) + 2^37 < 2^64 for S <= 51.
 template <int S, int G, bool LAZY = false>
 __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S], uint32_t np, uint32_t g,
                                          uint32_t *stage, uint32_t s = 0) {
-    static_assert(S <= 32 && (G == 4 || G == 8 || G == 16), "accumulator bound; groups tile a 16-lane DPP row");
+    static_assert(S <= 51 && (G == 4 || G == 8 || G == 16), "accumulator bound; groups tile a 16-lane DPP row");
     constexpr int ST = stage_stride(S);
     uint64_t t[S];
 #pragma unroll
@@ -320,21 +336,39 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S
         stage[g * ST + p] = x[p];
     }
     const bool top = (g == G - 1);
-    for (int o = 0; o < G; ++o) {
-        uint32_t xb[S];
+    if constexpr (S > 32) {
+        // Wide lanes (quads of 4247-bit numbers at two wavefronts per SIMD).
+        // Left to itself the compiler reassociates a block's rows into one
+        // dependent chain of multiply-adds per accumulator, which keeps every
+        // row's multiplier and m live and spills.  Here each row's multiplier
+        // is read one row ahead, a scheduling barrier keeps the rows in
+        // order, and every fourth row the accumulators pass through an empty
+        // asm, which bounds each chain at 8 products: no spills, and 2^15
+        // tests of 4247 bits take 204 ms against 218-220 without the pin.
+        uint32_t xnext = stage[0];
+        for (int o = 0; o < G; ++o) {
 #pragma unroll
-        for (int u = 0; u < S; ++u) xb[u] = stage[o * ST + u];
+            for (int u = 0; u < S; ++u) {
+                const uint32_t xi = xnext;
+                if (u + 1 < S)
+                    xnext = stage[o * ST + u + 1];
+                else if (o + 1 < G)
+                    xnext = stage[(o + 1) * ST];
+                cios_row<S, G>(t, x, n, xi, np, top, u);
+                __builtin_amdgcn_sched_barrier(0);
+                if (u % 4 == 3) {
 #pragma unroll
-        for (int u = 0; u < S; ++u) {
-            const uint32_t xi = xb[u];
+                    for (int p = 0; p < S; ++p) asm volatile("" : "+v"(t[p]));
+                }
+            }
+        }
+    } else {
+        for (int o = 0; o < G; ++o) {
+            uint32_t xb[S];
 #pragma unroll
-            for (int j = 0; j < S; ++j) t[(u + j) % S] = mad(xi, x[j], t[(u + j) % S]);
-            const uint32_t m = gbcast0<G>(((uint32_t)t[u] * np) & MASK);
+            for (int u = 0; u < S; ++u) xb[u] = stage[o * ST + u];
 #pragma unroll
-            for (int j = 0; j < S; ++j) t[(u + j) % S] = mad(m, n[j], t[(u + j) % S]);
-            t[(u + 1) % S] += t[u] >> RB;
-            const uint32_t lo = row_down((uint32_t)t[u] & MASK);
-            t[u] = top ? 0u : lo;
+            for (int u = 0; u < S; ++u) cios_row<S, G>(t, x, n, xb[u], np, top, u);
         }
     }
     if constexpr (LAZY)

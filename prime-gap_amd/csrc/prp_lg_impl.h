@@ -28,10 +28,10 @@ __device__ __forceinline__ uint32_t *group_stage(uint32_t *lds) {
 // lanes share the item), so DPP exchanges never read a finished lane of the
 // same group.
 template <int S, int G, int KIND>
-__global__ void __launch_bounds__(256) prp_lg_kernel(const uint32_t *__restrict__ n, uint32_t limbs, uint32_t count,
-                                                     const uint32_t *__restrict__ d_count,
-                                                     const uint32_t *__restrict__ d_idx, Bases bases,
-                                                     uint8_t *__restrict__ out) {
+__device__ __forceinline__ void prp_lg_item(const uint32_t *__restrict__ n, uint32_t limbs, uint32_t count,
+                                            const uint32_t *__restrict__ d_count,
+                                            const uint32_t *__restrict__ d_idx, Bases bases,
+                                            uint8_t *__restrict__ out) {
     extern __shared__ uint32_t lg_lds[];
     uint32_t *stage = group_stage<S, G>(lg_lds);
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
@@ -51,24 +51,52 @@ __global__ void __launch_bounds__(256) prp_lg_kernel(const uint32_t *__restrict_
 }
 
 template <int S, int G, int KIND>
+__global__ void __launch_bounds__(256) prp_lg_kernel(const uint32_t *__restrict__ n, uint32_t limbs, uint32_t count,
+                                                     const uint32_t *__restrict__ d_count,
+                                                     const uint32_t *__restrict__ d_idx, Bases bases,
+                                                     uint8_t *__restrict__ out) {
+    prp_lg_item<S, G, KIND>(n, limbs, count, d_count, d_idx, bases, out);
+}
+
+// Lanes of more than 32 digits (the quads of 4247-bit numbers): held to
+// exactly two wavefronts per SIMD, where x, N and the accumulators take 152
+// of a wavefront's 256 registers; left to choose, the compiler targets one
+// wavefront and moves registers through AGPRs.
+template <int S, int G, int KIND>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+prp_lg_wide_kernel(const uint32_t *__restrict__ n, uint32_t limbs, uint32_t count,
+                   const uint32_t *__restrict__ d_count, const uint32_t *__restrict__ d_idx, Bases bases,
+                   uint8_t *__restrict__ out) {
+    prp_lg_item<S, G, KIND>(n, limbs, count, d_count, d_idx, bases, out);
+}
+
+template <int S, int G, int KIND>
+constexpr auto prp_lg_fn() {
+    if constexpr (S > 32)
+        return &prp_lg_wide_kernel<S, G, KIND>;
+    else
+        return &prp_lg_kernel<S, G, KIND>;
+}
+
+template <int S, int G, int KIND>
 hipError_t launch_lg(const uint32_t *d_n, uint32_t limbs, uint32_t count, const uint32_t *d_count,
                      const uint32_t *d_idx, Bases bases, uint8_t *d_out, int blocks_cap, hipStream_t stream) {
     if (count == 0) return hipSuccess;
     if (KIND == KIND_SPRP && bases.n != 1) return hipErrorInvalidValue;
     const dim3 grid((uint32_t)(((size_t)count * G + 255) / 256));
-    const size_t lds = lg_lds_bytes(prp_lg_kernel<S, G, KIND>, S, blocks_cap);
-    hipLaunchKernelGGL((prp_lg_kernel<S, G, KIND>), grid, dim3(256), lds, stream, d_n, limbs, count, d_count, d_idx,
-                       bases, d_out);
+    constexpr auto kernel = prp_lg_fn<S, G, KIND>();
+    const size_t lds = lg_lds_bytes(kernel, S, blocks_cap);
+    hipLaunchKernelGGL(kernel, grid, dim3(256), lds, stream, d_n, limbs, count, d_count, d_idx, bases, d_out);
     return hipGetLastError();
 }
 
 // Streaming-scan round, a group per number: confirmation items padded to
 // whole wavefronts (64 / G items) first, then Fermat items.
 template <int S, int G>
-__global__ void __launch_bounds__(256) scan_lg_kernel(const uint32_t *__restrict__ n, uint32_t limbs,
-                                                      uint32_t conf_rows, uint32_t conf_max, uint32_t max_items,
-                                                      const uint32_t *__restrict__ ctr,
-                                                      const uint8_t *__restrict__ base, uint8_t *__restrict__ out) {
+__device__ __forceinline__ void scan_lg_item(const uint32_t *__restrict__ n, uint32_t limbs, uint32_t conf_rows,
+                                             uint32_t conf_max, uint32_t max_items,
+                                             const uint32_t *__restrict__ ctr, const uint8_t *__restrict__ base,
+                                             uint8_t *__restrict__ out) {
     extern __shared__ uint32_t lg_lds[];
     uint32_t *stage = group_stage<S, G>(lg_lds);
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
@@ -90,6 +118,30 @@ __global__ void __launch_bounds__(256) scan_lg_kernel(const uint32_t *__restrict
 }
 
 template <int S, int G>
+__global__ void __launch_bounds__(256) scan_lg_kernel(const uint32_t *__restrict__ n, uint32_t limbs,
+                                                      uint32_t conf_rows, uint32_t conf_max, uint32_t max_items,
+                                                      const uint32_t *__restrict__ ctr,
+                                                      const uint8_t *__restrict__ base, uint8_t *__restrict__ out) {
+    scan_lg_item<S, G>(n, limbs, conf_rows, conf_max, max_items, ctr, base, out);
+}
+
+template <int S, int G>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+scan_lg_wide_kernel(const uint32_t *__restrict__ n, uint32_t limbs, uint32_t conf_rows, uint32_t conf_max,
+                    uint32_t max_items, const uint32_t *__restrict__ ctr, const uint8_t *__restrict__ base,
+                    uint8_t *__restrict__ out) {
+    scan_lg_item<S, G>(n, limbs, conf_rows, conf_max, max_items, ctr, base, out);
+}
+
+template <int S, int G>
+constexpr auto scan_lg_fn() {
+    if constexpr (S > 32)
+        return &scan_lg_wide_kernel<S, G>;
+    else
+        return &scan_lg_kernel<S, G>;
+}
+
+template <int S, int G>
 hipError_t launch_scan_lg(const uint32_t *d_n, uint32_t limbs, uint32_t conf_rows, uint32_t conf_max,
                           uint32_t max_items, const uint32_t *d_ctr, const uint8_t *d_base, uint8_t *d_out,
                           int blocks_cap, hipStream_t stream) {
@@ -97,17 +149,19 @@ hipError_t launch_scan_lg(const uint32_t *d_n, uint32_t limbs, uint32_t conf_row
     const size_t items = ((size_t)conf_max + per_wave - 1) / per_wave * per_wave + max_items;
     if (items == 0) return hipSuccess;
     const dim3 grid((uint32_t)((items * G + 255) / 256));
-    const size_t lds = lg_lds_bytes(scan_lg_kernel<S, G>, S, blocks_cap);
-    hipLaunchKernelGGL((scan_lg_kernel<S, G>), grid, dim3(256), lds, stream, d_n, limbs, conf_rows, conf_max,
-                       max_items, d_ctr, d_base, d_out);
+    constexpr auto kernel = scan_lg_fn<S, G>();
+    const size_t lds = lg_lds_bytes(kernel, S, blocks_cap);
+    hipLaunchKernelGGL(kernel, grid, dim3(256), lds, stream, d_n, limbs, conf_rows, conf_max, max_items, d_ctr,
+                       d_base, d_out);
     return hipGetLastError();
 }
 
 template <int S, int G>
 int occupancy_scan_lg() {
     int n = 0;
-    const size_t lds = lg_lds_bytes(scan_lg_kernel<S, G>, S, 0);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_lg_kernel<S, G>, 256, lds) != hipSuccess || n <= 0)
+    constexpr auto kernel = scan_lg_fn<S, G>();
+    const size_t lds = lg_lds_bytes(kernel, S, 0);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, 256, lds) != hipSuccess || n <= 0)
         n = 1;
     return n;
 }
@@ -115,9 +169,9 @@ int occupancy_scan_lg() {
 template <int S, int G, int KIND>
 int occupancy_lg() {
     int n = 0;
-    const size_t lds = lg_lds_bytes(prp_lg_kernel<S, G, KIND>, S, 0);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, prp_lg_kernel<S, G, KIND>, 256, lds) != hipSuccess ||
-        n <= 0)
+    constexpr auto kernel = prp_lg_fn<S, G, KIND>();
+    const size_t lds = lg_lds_bytes(kernel, S, 0);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, 256, lds) != hipSuccess || n <= 0)
         n = 1;
     return n;
 }

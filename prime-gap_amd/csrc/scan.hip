@@ -1001,10 +1001,10 @@ class Scanner {
         const uint32_t sides = (uint32_t)std::min<uint64_t>(2ull * na, (uint64_t)last_scanning + 2ull * admitted_last);
         if (sides <= LATENCY_SIDES && !e->opt.lanes_per_number && !e->opt.radix) {
             // wide one-lane numbers move to 8 lanes (16 while very few sides
-            // scan), 8-lane numbers to 16
+            // scan), 4- and 8-lane numbers to 16
             const bool row = sides <= LATENCY_SIDES_ROW;
             const int lanes = cf.v->lanes == 1 && cf.v->digits >= LATENCY_MIN_DIGITS ? (row ? 16 : 8)
-                              : cf.v->lanes == 8                                       ? 16
+                              : cf.v->lanes == 4 || cf.v->lanes == 8                   ? 16
                                                                                        : 0;
             const Variant *lf = lanes ? find_variant(KIND_FERMAT2, (int)bits, 28, lanes) : nullptr;
             const Variant *lc = lanes && conf.n ? find_variant(KIND_SPRP, (int)bits, 28, lanes) : nullptr;

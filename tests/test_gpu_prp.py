@@ -121,7 +121,8 @@ def test_large_family_and_sprp(engine):
 @pytest.mark.parametrize("bits", [2300, 2490, 2600, 3000, 3320, 3500, 4100, 4250])
 def test_quad_variants_random_sizes(engine, bits):
     # wider than one lane holds: four lanes per number up to 3356 bits (radix
-    # 2^28, 120 digits), eight above
+    # 2^28, 120 digits), eight to 3580 (128 digits), four again to 4252 (152
+    # digits: 4 x 38 ties with 8 x 19 and the fewer lanes win), eight above
     rng = random.Random(bits)
     vals = [rng.getrandbits(bits) | (1 << (bits - 1)) | 1 for _ in range(96)]
     p = []
@@ -132,7 +133,28 @@ def test_quad_variants_random_sizes(engine, bits):
         p.append(x)
     got = check(engine, vals + p)
     assert got[-2:].all()
-    assert engine.last_launch().lanes_per_number == (4 if bits <= 28 * 120 - 4 else 8)
+    quad = bits <= 28 * 120 - 4 or 28 * 128 - 4 < bits <= 28 * 152 - 4
+    assert engine.last_launch().lanes_per_number == (4 if quad else 8)
+
+
+@pytest.mark.parametrize("bits", [3400, 4000, 4240])
+def test_quad_38_digit_lanes(bits):
+    # the widest quad variant (4 lanes x 38 digits, two wavefronts per SIMD,
+    # its own row schedule) against GMP, Fermat and strong PRP, with primes
+    # planted among random odd numbers
+    rng = random.Random(bits + 38)
+    vals = [rng.getrandbits(bits) | (1 << (bits - 1)) | 1 for _ in range(120)]
+    for v in vals[:2]:
+        x = v
+        while not ol.probab_prime(x):
+            x += 2
+        vals.append(x)
+    with pgap.Engine(0, lanes=4) as e:
+        got = check(e, vals)
+        assert got[-2:].all()
+        assert (e.last_launch().lanes_per_number, e.last_launch().digits) == (4, 152)
+        check(e, vals[:40] + vals[-2:], pgap.SPRP, (2, 3))
+        assert (e.last_launch().lanes_per_number, e.last_launch().digits) == (4, 152)
 
 
 @pytest.fixture(scope="module")
