@@ -22,7 +22,7 @@
 // A 4247-bit number (152 digits) runs on G = 4 lanes of S = 38 digits at
 // two wavefronts per SIMD (x, N and the accumulators take 152 registers per
 // lane; sqr_redc keeps its rows in order, see there), or on G = 8 lanes of
-// 19 digits at three.  The quad issues 14% fewer instructions per test (one
+// 19 digits at three.  The quad issues 17% fewer instructions per test (one
 // m broadcast per 76 products instead of per 38) and is the faster of the
 // two.  G = 16 (a whole DPP row) cuts a test's latency at a lower
 // throughput; the streaming scan uses it while few m's are in flight.
@@ -312,6 +312,28 @@ __device__ __forceinline__ void cios_row(uint64_t (&t)[S], const uint32_t (&x)[S
     t[u] = top ? 0u : lo;
 }
 
+// cios_row for the wide lanes (S > 32): m is broadcast by a DPP move with
+// no old value when the group is a quad (every lane has a source), and the
+// digit handed down is masked after the move by a per-lane mask that is
+// zero in the group's top lane, in place of a mask and a select: 7 other
+// instructions per row instead of 9 (1% of the 4247-bit kernel).
+template <int S, int G>
+__device__ __forceinline__ void cios_row_wide(uint64_t (&t)[S], const uint32_t (&x)[S], const uint32_t (&n)[S],
+                                              uint32_t xi, uint32_t np, uint32_t lane_mask, int u) {
+#pragma unroll
+    for (int j = 0; j < S; ++j) t[(u + j) % S] = mad(xi, x[j], t[(u + j) % S]);
+    const uint32_t mu = ((uint32_t)t[u] * np) & MASK;
+    uint32_t m;
+    if constexpr (G == 4)
+        m = (uint32_t)__builtin_amdgcn_mov_dpp((int)mu, 0 /* quad_perm [0,0,0,0] */, 0xF, 0xF, true);
+    else
+        m = gbcast0<G>(mu);
+#pragma unroll
+    for (int j = 0; j < S; ++j) t[(u + j) % S] = mad(m, n[j], t[(u + j) % S]);
+    t[(u + 1) % S] += t[u] >> RB;
+    t[u] = row_down((uint32_t)t[u]) & lane_mask;
+}
+
 // x <- x^2 R^{-1} mod N (lazy: x < 4N in, < 2N out), R = 2^(28*G*S).
 // `stage` is the group's LDS stage (G rows of stage_stride(S) words).
 // LAZY: the result comes as lazy_carry digits times 2^s (< 4N); otherwise as
@@ -345,6 +367,7 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S
         // order, and every fourth row the accumulators pass through an empty
         // asm, which bounds each chain at 8 products: no spills, and 2^15
         // tests of 4247 bits take 204 ms against 218-220 without the pin.
+        const uint32_t lane_mask = top ? 0u : MASK;
         uint32_t xnext = stage[0];
         for (int o = 0; o < G; ++o) {
 #pragma unroll
@@ -354,7 +377,7 @@ __device__ __forceinline__ void sqr_redc(uint32_t (&x)[S], const uint32_t (&n)[S
                     xnext = stage[o * ST + u + 1];
                 else if (o + 1 < G)
                     xnext = stage[(o + 1) * ST];
-                cios_row<S, G>(t, x, n, xi, np, top, u);
+                cios_row_wide<S, G>(t, x, n, xi, np, lane_mask, u);
                 __builtin_amdgcn_sched_barrier(0);
                 if (u % 4 == 3) {
 #pragma unroll
