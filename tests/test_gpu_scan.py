@@ -272,10 +272,11 @@ def test_scan_2100_bit_family_small():
             (want["prev_p"], want["next_p"], want["p_tests"], want["n_tests"]), m
 
 
-@pytest.mark.parametrize("lanes", [0, 8, 16])
+@pytest.mark.parametrize("lanes", [0, 4, 8, 16])
 def test_scan_4300_bit_family_small(lanes):
     # 3001#/2190 (N ~ 4231-4248 bits): lane-group variants in the scan (with
-    # lanes=0 two m's take latency-first rounds on the 16-lane variants)
+    # lanes=0 two m's take latency-first rounds on the 16-lane variants;
+    # lanes=4 runs them on the 4 x 38 quads that load rounds use)
     P, D, SL = 3001, 2190, 40000
     sv = ol.Sieve(P, D, SL, 1_000_000)
     reqs = [(m,) + sv(m) for m in (1, 7)]
