@@ -309,7 +309,9 @@ __device__ __forceinline__ void cios_row(uint64_t (&t)[S], const uint32_t (&x)[S
     for (int j = 0; j < S; ++j) t[(u + j) % S] = mad(m, n[j], t[(u + j) % S]);
     t[(u + 1) % S] += t[u] >> RB;
     const uint32_t lo = row_down((uint32_t)t[u] & MASK);
-    t[u] = top ? 0u : lo;
+    // a group of 16 is a whole DPP row, whose last lane the move already
+    // zeroes; a smaller group's top lane reads the next group's lane 0
+    t[u] = (G == 16 || !top) ? lo : 0u;
 }
 
 // cios_row for the wide lanes (S > 32): m is broadcast by a DPP move with
@@ -421,17 +423,7 @@ __device__ __forceinline__ void mul_redc(uint32_t (&x)[S], const uint32_t (&y)[S
 #pragma unroll
         for (int u = 0; u < S; ++u) xb[u] = stage[o * ST + u];
 #pragma unroll
-        for (int u = 0; u < S; ++u) {
-            const uint32_t xi = xb[u];
-#pragma unroll
-            for (int j = 0; j < S; ++j) t[(u + j) % S] = mad(xi, y[j], t[(u + j) % S]);
-            const uint32_t m = gbcast0<G>(((uint32_t)t[u] * np) & MASK);
-#pragma unroll
-            for (int j = 0; j < S; ++j) t[(u + j) % S] = mad(m, n[j], t[(u + j) % S]);
-            t[(u + 1) % S] += t[u] >> RB;
-            const uint32_t lo = row_down((uint32_t)t[u] & MASK);
-            t[u] = top ? 0u : lo;
-        }
+        for (int u = 0; u < S; ++u) cios_row<S, G>(t, y, n, xb[u], np, top, u);
     }
     normalize<S, G>(t, x, g);
 }
