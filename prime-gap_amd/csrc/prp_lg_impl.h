@@ -1,5 +1,6 @@
-// prp_lg_impl.h -- batch kernels for the lane-group radix-2^28 variants (G = 4
-// or 8 lanes per number); included by the prp_lg_*.hip instantiation units.
+// prp_lg_impl.h -- batch and scan kernels for the lane-group radix-2^28
+// variants (G = 4, 8 or 16 lanes per number) and the strong Lucas kernels;
+// included by the prp_lg_*.hip and lucas_lg_*.hip instantiation units.
 #pragma once
 #include <algorithm>
 #include "kernels.h"
