@@ -432,6 +432,9 @@ def main():
     ap.add_argument("--no-plain-503", action="store_true", help="skip the plain 503#/210 file scan")
     ap.add_argument("--mad-peak", action="store_true", help="only measure the v_mad_u64_u32 rate")
     ap.add_argument("--no-4300", action="store_true", help="skip the 4300-bit batch")
+    ap.add_argument("--batch-4300-lanes", type=int, default=0,
+                    help="lanes per number for the 4300-bit batch (0: the engine's choice, 4 lanes x 38 digits; "
+                         "8: the 8 x 19 variant)")
     ap.add_argument("--no-700", action="store_true", help="skip the 700-bit batch")
     ap.add_argument("--stream-m", type=int, default=16384)
     ap.add_argument("--stream-load", type=float, default=0.5)
@@ -585,7 +588,10 @@ def main():
     if not args.no_700:
         extra["batch_700_bit"] = bench_batch_700(engine, hipmem, pgap, 1 << 20, args.seed + 7, peak["mac_per_s"])
     if not args.no_4300:
-        extra["batch_4300_bit"] = bench_batch_4300(engine, hipmem, pgap, 1 << 15, args.seed + 4, peak["mac_per_s"])
+        e4 = pgap.Engine(0, lanes=args.batch_4300_lanes) if args.batch_4300_lanes else engine
+        extra["batch_4300_bit"] = bench_batch_4300(e4, hipmem, pgap, 1 << 15, args.seed + 4, peak["mac_per_s"])
+        if e4 is not engine:
+            e4.close()
     if not args.no_plain_503:
         # the reference's CPU-runnable case: 503#/210, SL=4000, every m < 2000
         # coprime to 210, unknowns sieved to 1e8 (a fresh engine: a scan's
