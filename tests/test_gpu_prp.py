@@ -157,6 +157,25 @@ def test_quad_38_digit_lanes(bits):
         assert (e.last_launch().lanes_per_number, e.last_launch().digits) == (4, 152)
 
 
+def test_quad_38_digit_lanes_at_their_bounds():
+    # the widest numbers the 4 x 38 variant takes: 28 * 152 - 4 = 4252 bits
+    # for Fermat (the squaring's lazy headroom), 4248 for strong PRP, all
+    # bits set at the top and a planted prime at each bound
+    rng = random.Random(4252)
+    with pgap.Engine(0, lanes=4) as e:
+        for bits, kind, bases in ((4252, pgap.FERMAT2, ()), (4248, pgap.SPRP, (2,))):
+            top = ((1 << bits) - 1) ^ ((1 << (bits - 40)) - 1)   # 40 top bits set
+            vals = [top | rng.getrandbits(bits - 40) | 1 for _ in range(48)]
+            x = vals[0]
+            while not ol.probab_prime(x):
+                x -= 2
+            vals.append(x)
+            assert all(v.bit_length() == bits for v in vals)
+            got = check(e, vals, kind, bases)
+            assert got[-1] == 1
+            assert (e.last_launch().lanes_per_number, e.last_launch().digits) == (4, 152)
+
+
 @pytest.fixture(scope="module")
 def engine8():
     e = pgap.Engine(0, lanes=8)   # force the 8-lanes-per-number variants
